@@ -1,0 +1,223 @@
+/*
+ * mgpusort.h -- C API of the MI355X (gfx950) LSD radix sort.
+ *
+ * Drop-in for the reference's inc/mgpusort.h (seanbaxter/mgpu_old): the same
+ * names, status codes, argument meanings and sortData_d layout, so existing
+ * C/C++ host code compiles and links against libmgpusort.so unchanged.  The
+ * engine loads gfx950 code objects (*.hsaco) from kernelPath through the HIP
+ * module API; no CUDA header is needed (CUdeviceptr is provided below as a
+ * 64-bit integer of the same size and alignment).
+ *
+ * Every function returns a status code; nothing throws across this boundary.
+ * An engine is bound to the HIP device that is current when it is created and
+ * is not reentrant: one host thread at a time per engine.
+ */
+#ifndef MGPUSORT_H
+#define MGPUSORT_H
+
+#define SORTAPI
+
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Everything declared here is exported from libmgpusort.so. */
+#if defined(__GNUC__)
+#pragma GCC visibility push(default)
+#endif
+
+/* Device pointer as an integer, the role CUdeviceptr (64-bit) plays in the
+ * reference header (inc/mgpusort.h:15,115-126). */
+typedef unsigned long long sortDevicePtr_t;
+#ifndef MGPUSORT_NO_CUDEVICEPTR
+typedef sortDevicePtr_t CUdeviceptr;
+#endif
+
+/* Status codes: inc/mgpusort.h:21-36, same order and values. */
+typedef enum {
+    SORT_STATUS_SUCCESS = 0,
+    SORT_STATUS_NOT_INITIALIZED,
+    SORT_STATUS_DEVICE_ALLOC_FAILED,
+    SORT_STATUS_HOST_ALLOC_FAILED,
+    SORT_STATUS_CONFIG_NOT_SUPPORTED,
+    SORT_STATUS_CONTEXT_MISMATCH,
+    SORT_STATUS_INVALID_CONTEXT,
+    SORT_STATUS_KERNEL_NOT_FOUND,
+    SORT_STATUS_KERNEL_ERROR,
+    SORT_STATUS_LAUNCH_ERROR,
+    SORT_STATUS_INVALID_VALUE,
+    SORT_STATUS_DEVICE_ERROR,
+    SORT_STATUS_INTERNAL_ERROR,
+    SORT_STATUS_UNSUPPORTED_DEVICE
+} sortStatus_t;
+
+/* valueCount that asks the first executed pass to write source indices into
+ * values1 (the reference spells it -1: inc/mgpusort.h:70-73). */
+#define SORT_VALUE_INDEX (-1)
+
+/* Largest numElements one sortArray call accepts (offsets are kept in 30 bits
+ * of a lookback word).  Larger arrays return SORT_STATUS_CONFIG_NOT_SUPPORTED. */
+#define SORT_MAX_ELEMENTS ((1 << 30) - 1)
+
+/* Replaces inc/mgpusort.h:39.  Returns 0 for an unknown code. */
+const char* SORTAPI sortStatusString(sortStatus_t status);
+
+struct sortEngine_d;
+typedef struct sortEngine_d* sortEngine_t;
+
+/* Replaces inc/mgpusort.h:50-51.  Binds to the current HIP device, which must
+ * be gfx950 (else SORT_STATUS_UNSUPPORTED_DEVICE), and loads the code objects
+ * kernelPath + "<name>.hsaco" (prefix concatenation, as the reference does
+ * with cubins).  A NULL or empty kernelPath means the directory holding
+ * libmgpusort.so; the environment variable MGPUSORT_KERNEL_PATH, when set,
+ * overrides both. */
+sortStatus_t SORTAPI sortCreateEngine(const char* kernelPath, sortEngine_t* engine);
+
+/* Replaces inc/mgpusort.h:54-55.  Validates the arguments as the reference
+ * does (128 or 256 threads, 4 or 8 values per thread); every kernel is
+ * already resident after sortCreateEngine, so nothing else happens. */
+sortStatus_t SORTAPI sortLoadKernel(sortEngine_t engine, int numSortThreads,
+                                    int valuesPerThread, bool useTransList, int valueCount);
+
+/* Replace inc/mgpusort.h:58-59: engine reference count. */
+sortStatus_t SORTAPI sortIncEngine(sortEngine_t engine);
+sortStatus_t SORTAPI sortReleaseEngine(sortEngine_t engine);
+
+/* Alias of sortReleaseEngine (named in a comment at
+ * sort/src/mgpusort/mgpusort.cpp:198 but never exported there). */
+sortStatus_t SORTAPI sortDestroyEngine(sortEngine_t engine);
+
+/* Replaces inc/mgpusort.h:63-147, same field order, LP64 layout
+ * (sizeof == 144, pointers from offset 32). */
+typedef struct {
+    /* Capacity of every buffer, in elements. */
+    int maxElements;
+    /* Number of elements to sort. */
+    int numElements;
+    /* Number of value arrays carried (0..6), or SORT_VALUE_INDEX (-1) to write
+     * source indices into values1 on the first pass and carry them after. */
+    int valueCount;
+    /* Sort by the bit field [firstBit, endBit) of each key. */
+    int firstBit;
+    int endBit;
+    /* Accepted for compatibility.  This engine never reads or writes any
+     * element at or beyond numElements, so the keys past the end are always
+     * preserved. */
+    bool preserveEndKeys;
+    /* Look for already-sorted input and skip the passes that would not change
+     * the order; the output is the same either way. */
+    bool earlyExit;
+    /* Toggled once per executed pass (buffer swap).  An array at [0] before
+     * the sort is at [parity] after it; [0] always holds the sorted result. */
+    int parity;
+    union {
+        struct {
+            CUdeviceptr keys[2];
+            union {
+                struct {
+                    CUdeviceptr values1[2];
+                    CUdeviceptr values2[2];
+                    CUdeviceptr values3[2];
+                    CUdeviceptr values4[2];
+                    CUdeviceptr values5[2];
+                    CUdeviceptr values6[2];
+                };
+                CUdeviceptr values[6][2];
+            };
+        };
+        struct {
+            unsigned int* pKeys[2];
+            union {
+                struct {
+                    unsigned int* pValues1[2];
+                    unsigned int* pValues2[2];
+                    unsigned int* pValues3[2];
+                    unsigned int* pValues4[2];
+                    unsigned int* pValues5[2];
+                    unsigned int* pValues6[2];
+                };
+                unsigned int* pValues[6][2];
+            };
+        };
+    };
+} sortData_d;
+typedef sortData_d* sortData_t;
+
+/* Replace inc/mgpusort.h:155-157.  sortCreateData allocates the descriptor
+ * and all its buffers (maxElements rounded up to 2048, numElements =
+ * maxElements, bits [0, 32)); sortDestroyData frees both. */
+sortStatus_t SORTAPI sortCreateData(sortEngine_t engine, int maxElements, int valueCount,
+                                    sortData_t* data);
+sortStatus_t SORTAPI sortDestroyData(sortData_t data);
+
+/* Replace inc/mgpusort.h:164-165.  sortAllocData allocates every null buffer
+ * among keys[2] and the first (valueCount == -1 ? 1 : valueCount) value
+ * pairs, RoundUp(maxElements, 2048) elements each, and sets parity = 0.
+ * sortFreeData frees every non-null buffer (attached ones included) and
+ * zeroes the descriptor. */
+sortStatus_t SORTAPI sortAllocData(sortEngine_t engine, sortData_t data);
+sortStatus_t SORTAPI sortFreeData(sortData_d* data);
+
+/* Replaces inc/mgpusort.h:167.  Stable sort of keys[0][0, numElements) by the
+ * bit field [firstBit, endBit), values following the same permutation.  The
+ * work is queued on the engine's stream (the null stream unless
+ * sortSetStream chose another); the call returns once it is queued. */
+sortStatus_t SORTAPI sortArray(sortEngine_t engine, sortData_t data);
+
+/* Replaces inc/mgpusort.h:169-170.  Same as sortArray with the pass split
+ * chosen by the caller: ceil(bits / bitPass) passes of bitPass bits (the first
+ * ones a bit narrower when it does not divide), bitPass outside 1..6 taken as
+ * 6.  numSortThreads must be 128 or 256 and valuesPerThread 8, as in the
+ * reference; the engine picks its own tile shape. */
+sortStatus_t SORTAPI sortArrayEx(sortEngine_t engine, sortData_t data, int numSortThreads,
+                                 int valuesPerThread, int bitPass, bool useTransList);
+
+/* Replace inc/mgpusort.h:183-187.  Convenience wrappers (keys plus at most one
+ * value array, bits [0, numBits)); both block until the result is in place. */
+sortStatus_t SORTAPI sortHost(sortEngine_t engine, unsigned int* keys, unsigned int* values,
+                              int numElements, int numBits);
+sortStatus_t SORTAPI sortDevice(sortEngine_t engine, CUdeviceptr keys, CUdeviceptr values,
+                                int numElements, int numBits);
+
+/* ---- Additions with no counterpart in the reference. ---------------------- */
+
+/* Stream the engine queues its work on (a hipStream_t; NULL = null stream). */
+sortStatus_t SORTAPI sortSetStream(sortEngine_t engine, void* stream);
+sortStatus_t SORTAPI sortGetStream(sortEngine_t engine, void** stream);
+
+/* Number of digit passes sortArray runs for a bit range of numBits bits, and
+ * their widths (widths may be NULL).  Host-only; needs no device. */
+int SORTAPI sortPlanPasses(int numBits, int* widths);
+
+/* Waits for the engine's stream and reports SORT_STATUS_KERNEL_ERROR if a
+ * kernel detected an inconsistency (a scatter offset out of range, or a
+ * lookback wait that gave up) since the last call; SUCCESS otherwise. */
+sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine);
+
+/* Per-kernel timing: while enabled, every launch is bracketed by hipEvents
+ * on the engine's stream.  sortReadTiming waits for the stream, fills up to
+ * maxEntries aggregates (one per kernel name, in first-launch order), clears
+ * the record and returns the number of kernels seen. */
+typedef struct {
+    char name[64];
+    int launches;
+    double totalMs;
+    double minMs;
+    double maxMs;
+} sortKernelTiming_t;
+sortStatus_t SORTAPI sortEnableTiming(sortEngine_t engine, bool enable);
+int SORTAPI sortReadTiming(sortEngine_t engine, sortKernelTiming_t* out, int maxEntries);
+
+#if defined(__GNUC__)
+#pragma GCC visibility pop
+#endif
+
+#ifdef __cplusplus
+}  /* extern "C" */
+#endif
+
+#endif /* MGPUSORT_H */

@@ -1,0 +1,54 @@
+"""Build recipe of the native parts: gfx950 code objects and libmgpusort.so.
+
+Outputs go to mgpu_old_amd/lib/ (git-ignored, shipped with the tree):
+  mgpusort_radix.hsaco   digit_count, digit_scan, rank_scatter kernels
+  libmgpusort.so         the C ABI of include/mgpusort.h (host C++ over HIP)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.environ.get("HIPCC", os.path.join(ROCM, "bin", "hipcc"))
+CXX = os.environ.get("CXX", "g++")
+ARCH = "gfx950"
+# Packed FP32 instructions stay off: none of this code wants them, and some of
+# their forms are unreliable on the target boxes.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+CODE_OBJECTS = {
+    "mgpusort_radix.hsaco": ["radixsort.hip"],
+}
+HOST_SOURCES = ["engine.cpp", "loader.cpp"]
+
+
+def _run(cmd):
+    print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(LIB, exist_ok=True)
+    for obj, srcs in CODE_OBJECTS.items():
+        # A plain gfx950 ELF code object (no offload bundle) for hipModuleLoad.
+        _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "--offload-device-only",
+              "--no-gpu-bundle-output", *DEVICE_FLAGS,
+              "-I", CSRC, *[os.path.join(CSRC, s) for s in srcs],
+              "-o", os.path.join(LIB, obj)])
+    # Host code only: plain g++ against the HIP runtime, no device pass.
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+          "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__",
+          "-I", CSRC, "-I", INCLUDE, "-I", os.path.join(ROCM, "include"),
+          *[os.path.join(CSRC, s) for s in HOST_SOURCES],
+          "-o", os.path.join(LIB, "libmgpusort.so"),
+          "-L", os.path.join(ROCM, "lib"), f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}",
+          "-lamdhip64", "-ldl", "-pthread"])
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

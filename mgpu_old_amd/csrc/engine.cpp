@@ -1,0 +1,600 @@
+// libmgpusort engine: the C ABI of include/mgpusort.h over the HIP module API.
+//
+// Reference: sort/src/mgpusort/mgpusort.cpp (engine, pass loop, data
+// management) and sortdll.h (engine state).  What is kept from the reference
+// and why is listed beside each function; what differs:
+//   * passes are 8-bit digits (4 passes for 32-bit keys) chosen by
+//     sortPlanPasses instead of the Fermi tables of sorttables.cpp;
+//   * every pass is one rank_scatter launch after a single digit_count +
+//     digit_scan per sort (the reference runs count + 3 hist + sort per pass,
+//     mgpusort.cpp:424-516);
+//   * keys past numElements are never read or written, so there is no end-key
+//     save/set/restore (mgpusort.cpp:402-419,591-598);
+//   * pass errors are returned instead of swallowed (mgpusort.cpp:583,600).
+#include <hip/hip_runtime.h>
+
+#include <dlfcn.h>
+
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mgpusort.h"
+#include "kernel_args.h"
+#include "engine_internal.h"
+
+namespace {
+
+const char* const kStatusStrings[] = {
+    "SORT_STATUS_SUCCESS",          "SORT_STATUS_NOT_INITIALIZED",
+    "SORT_STATUS_DEVICE_ALLOC_FAILED", "SORT_STATUS_HOST_ALLOC_FAILED",
+    "SORT_STATUS_CONFIG_NOT_SUPPORTED", "SORT_STATUS_CONTEXT_MISMATCH",
+    "SORT_STATUS_INVALID_CONTEXT",  "SORT_STATUS_KERNEL_NOT_FOUND",
+    "SORT_STATUS_KERNEL_ERROR",     "SORT_STATUS_LAUNCH_ERROR",
+    "SORT_STATUS_INVALID_VALUE",    "SORT_STATUS_DEVICE_ERROR",
+    "SORT_STATUS_INTERNAL_ERROR",   "SORT_STATUS_UNSUPPORTED_DEVICE"};
+
+constexpr int kAllocRound = 2048;  // buffer rounding of mgpusort.cpp:663,696
+
+inline int round_up(int x, int r) { return ((x + r - 1) / r) * r; }
+
+std::string library_dir() {
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void*>(&sortCreateEngine), &info) && info.dli_fname) {
+        std::string p(info.dli_fname);
+        size_t s = p.rfind('/');
+        if (s != std::string::npos) return p.substr(0, s + 1);
+    }
+    return std::string("./");
+}
+
+// Scratch that grows only (AllocSortResources, mgpusort.cpp:319-343).
+sortStatus_t ensure_lookback(sortEngine_d* e, uint64_t words) {
+    if (words <= e->lookbackWords) return SORT_STATUS_SUCCESS;
+    if (e->lookback) {
+        if (hipStreamSynchronize(e->stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+        (void)hipFree(e->lookback);
+        e->lookback = nullptr;
+        e->lookbackWords = 0;
+    }
+    uint64_t alloc = words + (words >> 3) + 1024;  // headroom against regrowth
+    if (hipMalloc(&e->lookback, alloc * sizeof(uint32_t)) != hipSuccess) {
+        e->lookback = nullptr;
+        return SORT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    e->lookbackWords = alloc & ~uint64_t(3);
+    return SORT_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Internal helpers shared with the segmented path.
+
+int mgpu_plan_passes(int numBits, int maxWidth, int* widths) {
+    if (numBits <= 0) return 0;
+    if (maxWidth < 1) maxWidth = 1;
+    if (maxWidth > MGPU_RADIX_BITS) maxWidth = MGPU_RADIX_BITS;
+    int passes = (numBits + maxWidth - 1) / maxWidth;
+    int base = numBits / passes, extra = numBits % passes;
+    // Narrower passes first, as sortArrayEx orders them (mgpusort.cpp:638-642).
+    for (int p = 0; p < passes; ++p)
+        if (widths) widths[p] = base + (p >= passes - extra ? 1 : 0);
+    return passes;
+}
+
+namespace {
+
+hipEvent_t pooled_event(sortEngine_d* e) {
+    if (!e->eventPool.empty()) {
+        hipEvent_t ev = e->eventPool.back();
+        e->eventPool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev = nullptr;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    return ev;
+}
+
+}  // namespace
+
+sortStatus_t mgpu_launch(sortEngine_d* e, hipFunction_t f, unsigned grid, unsigned block,
+                         void* args, size_t argBytes) {
+    (void)argBytes;
+    void* params[] = {args};
+    hipEvent_t start = nullptr, stop = nullptr;
+    if (e->timing) {
+        start = pooled_event(e);
+        stop = pooled_event(e);
+        if (!start || !stop || hipEventRecord(start, e->stream) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+    }
+    hipError_t r = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, e->stream, params,
+                                         nullptr);
+    if (r != hipSuccess) return SORT_STATUS_LAUNCH_ERROR;
+    if (e->timing) {
+        if (hipEventRecord(stop, e->stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+        e->timed.push_back({f, start, stop});
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+// One sort of n keys by [firstBit, firstBit + sum(widths)) from in-buffers to
+// out-buffers alternating per pass: pass p reads keys[p&1]-side pointers.
+// Pointers are passed per pass by the caller through `bufs`.
+sortStatus_t mgpu_sort_passes(sortEngine_d* e, uint32_t n, int firstBit, int numPasses,
+                              const int* widths, int valueCount, uint32_t* const keyBuf[2],
+                              uint32_t* const valBuf[MGPU_MAX_VALUES][2]) {
+    if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
+    if (numPasses > MGPU_MAX_PASSES) return SORT_STATUS_INTERNAL_ERROR;
+    const SortConfig& cfg = e->config;
+    const uint32_t tile = cfg.blockThreads * cfg.itemsPerThread;
+    const uint32_t numTiles = (n + tile - 1) / tile;
+    const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
+    sortStatus_t st = ensure_lookback(e, lbWords);
+    if (st) return st;
+
+    if (e->histDirty) {
+        if (hipMemsetAsync(e->hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
+                           e->stream) != hipSuccess ||
+            hipMemsetAsync(e->tileCounters, 0, MGPU_MAX_PASSES * sizeof(uint32_t), e->stream) !=
+                hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        e->histDirty = false;
+    }
+
+    // digit_count
+    mgpuCountArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    ca.keys = keyBuf[0];
+    ca.hist = e->hist;
+    ca.clear = e->lookback;
+    ca.clear_words = (lbWords + 3) & ~uint64_t(3);
+    ca.n = n;
+    ca.num_passes = (uint32_t)numPasses;
+    ca.vec_ok = (reinterpret_cast<uintptr_t>(keyBuf[0]) & 15) == 0;
+    {
+        int bit = firstBit;
+        for (int p = 0; p < numPasses; ++p) {
+            ca.shift[p] = (uint32_t)bit;
+            ca.mask[p] = (1u << widths[p]) - 1u;
+            bit += widths[p];
+        }
+    }
+    uint32_t grid = (n + 8191) / 8192;
+    uint32_t maxGrid = (uint32_t)e->numCUs * 4;
+    if (grid > maxGrid) grid = maxGrid;
+    if (grid < 1) grid = 1;
+    ca.chunk = (uint32_t)(((uint64_t)n + grid - 1) / grid);
+    ca.chunk = (ca.chunk + 1023) & ~1023u;
+    grid = (n + ca.chunk - 1) / ca.chunk;
+    // Enough workgroups to zero the lookback words quickly as well.
+    uint32_t clearGrid = (uint32_t)((ca.clear_words / 4 + 256 * 16 - 1) / (256 * 16));
+    if (clearGrid > maxGrid) clearGrid = maxGrid;
+    if (clearGrid > grid) grid = clearGrid;
+    e->histDirty = true;
+    st = mgpu_launch(e, e->fnCount, grid, 256, &ca, sizeof(ca));
+    if (st) return st;
+
+    // digit_scan
+    mgpuScanArgs sa;
+    sa.hist = e->hist;
+    sa.digit_start = e->digitStart;
+    sa.tile_counters = e->tileCounters;
+    sa.num_passes = (uint32_t)numPasses;
+    st = mgpu_launch(e, e->fnScan, 1, 256, &sa, sizeof(sa));
+    if (st) return st;
+    e->histDirty = false;
+
+    // rank_scatter per pass
+    int bit = firstBit;
+    for (int p = 0; p < numPasses; ++p) {
+        mgpuPassArgs pa;
+        std::memset(&pa, 0, sizeof(pa));
+        const int in = p & 1, out = in ^ 1;
+        pa.keys_in = keyBuf[in];
+        pa.keys_out = keyBuf[out];
+        int mode;
+        if (valueCount == 0) mode = MGPU_VALUES_NONE;
+        else if (valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
+        else if (valueCount == 1) mode = MGPU_VALUES_SINGLE;
+        else mode = MGPU_VALUES_MULTI;
+        int carried = valueCount == -1 ? 1 : valueCount;
+        for (int v = 0; v < carried; ++v) {
+            pa.vals_in[v] = valBuf[v][in];
+            pa.vals_out[v] = valBuf[v][out];
+        }
+        pa.digit_start = e->digitStart + p * MGPU_RADIX;
+        pa.lookback = e->lookback + (uint64_t)p * numTiles * MGPU_RADIX;
+        pa.tile_counter = e->tileCounters + p;
+        pa.error_flag = e->errorFlag;
+        pa.n = n;
+        pa.num_tiles = numTiles;
+        pa.shift = (uint32_t)bit;
+        pa.mask = (1u << widths[p]) - 1u;
+        pa.num_values = (uint32_t)carried;
+        st = mgpu_launch(e, e->fnPass[mode], numTiles, cfg.blockThreads, &pa, sizeof(pa));
+        if (st) return st;
+        bit += widths[p];
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+namespace {
+
+// Validation shared by sortArray and sortArrayEx (mgpusort.cpp:360-364,
+// 609-611, 625).
+sortStatus_t validate(sortEngine_t engine, sortData_t data) {
+    if (!engine || !data) return SORT_STATUS_INVALID_VALUE;
+    if (data->numElements < 0 || data->numElements > data->maxElements)
+        return SORT_STATUS_INVALID_VALUE;
+    if (data->firstBit < 0 || data->endBit > 32 || data->endBit < data->firstBit)
+        return SORT_STATUS_INVALID_VALUE;
+    if (data->valueCount < -1 || data->valueCount > 6) return SORT_STATUS_INVALID_VALUE;
+    if (data->numElements > SORT_MAX_ELEMENTS) return SORT_STATUS_CONFIG_NOT_SUPPORTED;
+    return SORT_STATUS_SUCCESS;
+}
+
+// sortArrayFromList (mgpusort.cpp:560-601) without the end-key handling:
+// runs the passes, swaps every buffer pair once per executed pass and toggles
+// parity, exactly as sortPass does (mgpusort.cpp:518-526).
+sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, const int* widths) {
+    const uint32_t n = (uint32_t)data->numElements;
+    if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
+    const int carried = data->valueCount == -1 ? 1 : data->valueCount;
+    if (!data->keys[0] || !data->keys[1]) return SORT_STATUS_INVALID_VALUE;
+    // Index generation reads no values on its first pass, so values1[0] is
+    // needed only as the target of a second pass.
+    for (int v = 0; v < carried; ++v)
+        if (!data->values[v][1] ||
+            (!data->values[v][0] && (data->valueCount != -1 || numPasses > 1)))
+            return SORT_STATUS_INVALID_VALUE;
+
+    uint32_t* keyBuf[2] = {reinterpret_cast<uint32_t*>(data->keys[0]),
+                           reinterpret_cast<uint32_t*>(data->keys[1])};
+    uint32_t* valBuf[MGPU_MAX_VALUES][2] = {};
+    for (int v = 0; v < MGPU_MAX_VALUES; ++v)
+        for (int s = 0; s < 2; ++s) valBuf[v][s] = reinterpret_cast<uint32_t*>(data->values[v][s]);
+
+    sortStatus_t st = mgpu_sort_passes(e, n, data->firstBit, numPasses, widths, data->valueCount,
+                                       keyBuf, valBuf);
+    if (st) return st;
+    if (numPasses & 1) {
+        std::swap(data->keys[0], data->keys[1]);
+        for (int v = 0; v < MGPU_MAX_VALUES; ++v) std::swap(data->values[v][0], data->values[v][1]);
+    }
+    data->parity ^= (numPasses & 1);
+    return SORT_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exported C ABI.
+
+extern "C" {
+
+const char* SORTAPI sortStatusString(sortStatus_t status) {
+    int code = (int)status;
+    if (code < 0 || code >= (int)(sizeof(kStatusStrings) / sizeof(kStatusStrings[0]))) return 0;
+    return kStatusStrings[code];
+}
+
+int SORTAPI sortPlanPasses(int numBits, int* widths) {
+    if (numBits < 0 || numBits > 32) return 0;
+    return mgpu_plan_passes(numBits, MGPU_RADIX_BITS, widths);
+}
+
+// sortCreateEngine (mgpusort.cpp:200-250): attach to the current device,
+// check it, load the code objects, allocate the small fixed scratch.
+sortStatus_t SORTAPI sortCreateEngine(const char* kernelPath, sortEngine_t* engine) {
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    *engine = nullptr;
+    int device = -1;
+    if (hipGetDevice(&device) != hipSuccess || device < 0) return SORT_STATUS_INVALID_CONTEXT;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return SORT_STATUS_INVALID_CONTEXT;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return SORT_STATUS_UNSUPPORTED_DEVICE;
+
+    sortEngine_d* e = new (std::nothrow) sortEngine_d();
+    if (!e) return SORT_STATUS_HOST_ALLOC_FAILED;
+    e->device = device;
+    e->numCUs = prop.multiProcessorCount;
+    const char* env = std::getenv("MGPUSORT_KERNEL_PATH");
+    if (env && *env) e->kernelPath = env;
+    else if (kernelPath && *kernelPath) e->kernelPath = kernelPath;
+    else e->kernelPath = library_dir();
+
+    sortStatus_t st = mgpu_engine_load(e);
+    if (st) {
+        mgpu_engine_destroy(e);
+        return st;
+    }
+    *engine = e;
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortLoadKernel(sortEngine_t engine, int numSortThreads, int valuesPerThread,
+                                    bool useTransList, int valueCount) {
+    (void)useTransList;
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    // Same acceptance as LoadKernels (mgpusort.cpp:148-153,182-184).
+    if (numSortThreads != 128 && numSortThreads != 256) return SORT_STATUS_INVALID_VALUE;
+    if (valuesPerThread != 4 && valuesPerThread != 8) return SORT_STATUS_INVALID_VALUE;
+    if (valueCount < -1 || valueCount > 6) return SORT_STATUS_INVALID_VALUE;
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortIncEngine(sortEngine_t engine) {
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    ++engine->refcount;
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortReleaseEngine(sortEngine_t engine) {
+    if (engine && --engine->refcount == 0) mgpu_engine_destroy(engine);
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortDestroyEngine(sortEngine_t engine) { return sortReleaseEngine(engine); }
+
+sortStatus_t SORTAPI sortSetStream(sortEngine_t engine, void* stream) {
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    engine->stream = reinterpret_cast<hipStream_t>(stream);
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine) {
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    if (hipStreamSynchronize(engine->stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    uint32_t flag = 0;
+    if (hipMemcpy(&flag, engine->errorFlag, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    if (!flag) return SORT_STATUS_SUCCESS;
+    if (hipMemset(engine->errorFlag, 0, sizeof(uint32_t)) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    return SORT_STATUS_KERNEL_ERROR;
+}
+
+sortStatus_t SORTAPI sortEnableTiming(sortEngine_t engine, bool enable) {
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    engine->timing = enable;
+    return SORT_STATUS_SUCCESS;
+}
+
+int SORTAPI sortReadTiming(sortEngine_t engine, sortKernelTiming_t* out, int maxEntries) {
+    if (!engine) return -1;
+    if (hipStreamSynchronize(engine->stream) != hipSuccess) return -1;
+    std::vector<sortKernelTiming_t> agg;
+    std::map<hipFunction_t, size_t> slot;
+    for (auto& t : engine->timed) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, t.start, t.stop) != hipSuccess) ms = -1.f;
+        auto it = slot.find(t.fn);
+        if (it == slot.end()) {
+            sortKernelTiming_t k;
+            std::memset(&k, 0, sizeof(k));
+            std::strncpy(k.name, engine->kernelNames[t.fn].c_str(), sizeof(k.name) - 1);
+            k.minMs = 1e30;
+            it = slot.emplace(t.fn, agg.size()).first;
+            agg.push_back(k);
+        }
+        sortKernelTiming_t& k = agg[it->second];
+        k.launches += 1;
+        k.totalMs += ms;
+        if (ms < k.minMs) k.minMs = ms;
+        if (ms > k.maxMs) k.maxMs = ms;
+        engine->eventPool.push_back(t.start);
+        engine->eventPool.push_back(t.stop);
+    }
+    engine->timed.clear();
+    for (int i = 0; i < (int)agg.size() && i < maxEntries && out; ++i) out[i] = agg[i];
+    return (int)agg.size();
+}
+
+sortStatus_t SORTAPI sortGetStream(sortEngine_t engine, void** stream) {
+    if (!engine || !stream) return SORT_STATUS_INVALID_VALUE;
+    *stream = reinterpret_cast<void*>(engine->stream);
+    return SORT_STATUS_SUCCESS;
+}
+
+// sortArray (mgpusort.cpp:607-616).
+sortStatus_t SORTAPI sortArray(sortEngine_t engine, sortData_t data) {
+    if (!engine || !data) return SORT_STATUS_INVALID_VALUE;
+    int numBits = data->endBit - data->firstBit;
+    if (!numBits) return SORT_STATUS_SUCCESS;  // as mgpusort.cpp:610
+    sortStatus_t st = validate(engine, data);
+    if (st) return st;
+    int widths[MGPU_MAX_PASSES];
+    int passes = mgpu_plan_passes(numBits, MGPU_RADIX_BITS, widths);
+    return sort_with_widths(engine, data, passes, widths);
+}
+
+// sortArrayEx (mgpusort.cpp:618-648): the caller's digit width sets the pass
+// count; the widths are split evenly with narrower passes first.
+sortStatus_t SORTAPI sortArrayEx(sortEngine_t engine, sortData_t data, int numSortThreads,
+                                 int valuesPerThread, int bitPass, bool useTransList) {
+    (void)useTransList;
+    if (!engine || !data) return SORT_STATUS_INVALID_VALUE;
+    if ((numSortThreads != 128 && numSortThreads != 256) || valuesPerThread != 8)
+        return SORT_STATUS_INVALID_VALUE;
+    if (data->numElements > data->maxElements) return SORT_STATUS_INVALID_VALUE;
+    if (bitPass <= 0 || bitPass > 6) bitPass = 6;
+    int numBits = data->endBit - data->firstBit;
+    if (!numBits) return SORT_STATUS_SUCCESS;
+    sortStatus_t st = validate(engine, data);
+    if (st) return st;
+    if (bitPass > numBits) bitPass = numBits;
+    int widths[32];
+    int passes = mgpu_plan_passes(numBits, bitPass, widths);
+    // More than MGPU_MAX_PASSES passes: run them in groups (each group one
+    // digit_count + scan); parity and swaps stay per pass.
+    int done = 0, bit = data->firstBit;
+    while (done < passes) {
+        int group = passes - done < MGPU_MAX_PASSES ? passes - done : MGPU_MAX_PASSES;
+        int savedFirst = data->firstBit;
+        int savedValueCount = data->valueCount;
+        data->firstBit = bit;
+        // Index generation only on the first executed pass (GetSortCode,
+        // mgpusort.cpp:553); later groups carry the indices as one value.
+        if (done > 0 && savedValueCount == -1) data->valueCount = 1;
+        st = sort_with_widths(engine, data, group, widths + done);
+        data->firstBit = savedFirst;
+        data->valueCount = savedValueCount;
+        if (st) return st;
+        for (int p = 0; p < group; ++p) bit += widths[done + p];
+        done += group;
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+// sortCreateData (mgpusort.cpp:655-682).  Unlike the reference, a failed
+// allocation is returned (and nothing is leaked).
+sortStatus_t SORTAPI sortCreateData(sortEngine_t engine, int maxElements, int valueCount,
+                                    sortData_t* data) {
+    if (!data) return SORT_STATUS_INVALID_VALUE;
+    *data = nullptr;
+    if (valueCount > 6 || valueCount < -1 || maxElements < 0) return SORT_STATUS_INVALID_VALUE;
+    sortData_d* d = new (std::nothrow) sortData_d;
+    if (!d) return SORT_STATUS_HOST_ALLOC_FAILED;
+    std::memset(d, 0, sizeof(*d));
+    d->maxElements = round_up(maxElements, kAllocRound);
+    d->numElements = maxElements;
+    d->valueCount = valueCount;
+    d->firstBit = 0;
+    d->endBit = 32;
+    sortStatus_t st = sortAllocData(engine, d);
+    if (st) {
+        sortFreeData(d);
+        delete d;
+        return st;
+    }
+    *data = d;
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortDestroyData(sortData_t data) {
+    if (!data) return SORT_STATUS_SUCCESS;
+    sortFreeData(data);
+    delete data;
+    return SORT_STATUS_SUCCESS;
+}
+
+// sortAllocData (mgpusort.cpp:692-719).
+sortStatus_t SORTAPI sortAllocData(sortEngine_t engine, sortData_t data) {
+    if (!data) return SORT_STATUS_INVALID_VALUE;
+    (void)engine;
+    if (data->valueCount > 6 || data->valueCount < -1 || data->maxElements < 0)
+        return SORT_STATUS_INVALID_VALUE;
+    size_t elems = (size_t)round_up(data->maxElements, kAllocRound);
+    if (!elems) elems = kAllocRound;
+    int count = data->valueCount == -1 ? 1 : data->valueCount;
+    CUdeviceptr* ptrs = data->keys;  // keys[2] then values[6][2], contiguous
+    CUdeviceptr fresh[14] = {};
+    for (int i = 0; i < 2 + 2 * count; ++i) {
+        if (ptrs[i]) continue;
+        void* p = nullptr;
+        if (hipMalloc(&p, elems * sizeof(uint32_t)) != hipSuccess) {
+            for (int j = 0; j < i; ++j)
+                if (fresh[j]) (void)hipFree(reinterpret_cast<void*>(fresh[j]));
+            return SORT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+        fresh[i] = reinterpret_cast<CUdeviceptr>(p);
+    }
+    for (int i = 0; i < 14; ++i)
+        if (fresh[i]) ptrs[i] = fresh[i];
+    data->parity = 0;
+    return SORT_STATUS_SUCCESS;
+}
+
+// sortFreeData (mgpusort.cpp:721-734): frees every non-null buffer.
+sortStatus_t SORTAPI sortFreeData(sortData_d* data) {
+    if (!data) return SORT_STATUS_INVALID_VALUE;
+    CUdeviceptr* ptrs = data->keys;
+    for (int i = 0; i < 14; ++i)
+        if (ptrs[i]) (void)hipFree(reinterpret_cast<void*>(ptrs[i]));
+    std::memset(data, 0, sizeof(sortData_d));
+    return SORT_STATUS_SUCCESS;
+}
+
+// sortHost (mgpusort.cpp:740-776), copying 4 bytes per element (the reference
+// copies sizeof(double) per element, past the end of both buffers).
+sortStatus_t SORTAPI sortHost(sortEngine_t engine, unsigned int* keys, unsigned int* values,
+                              int numElements, int numBits) {
+    if (!engine || !keys || numElements < 0 || numBits < 0 || numBits > 32)
+        return SORT_STATUS_INVALID_VALUE;
+    sortData_d d;
+    std::memset(&d, 0, sizeof(d));
+    d.maxElements = numElements;
+    d.numElements = numElements;
+    d.valueCount = values ? 1 : 0;
+    sortStatus_t st = sortAllocData(engine, &d);
+    if (st) return st;
+    d.endBit = numBits;
+    size_t bytes = sizeof(uint32_t) * (size_t)numElements;
+    hipStream_t s = engine->stream;
+    do {
+        if (hipMemcpyAsync(reinterpret_cast<void*>(d.keys[0]), keys, bytes, hipMemcpyHostToDevice,
+                           s) != hipSuccess) { st = SORT_STATUS_DEVICE_ERROR; break; }
+        if (values && hipMemcpyAsync(reinterpret_cast<void*>(d.values1[0]), values, bytes,
+                                     hipMemcpyHostToDevice, s) != hipSuccess) {
+            st = SORT_STATUS_DEVICE_ERROR; break;
+        }
+        st = sortArray(engine, &d);
+        if (st) break;
+        if (hipMemcpyAsync(keys, reinterpret_cast<void*>(d.keys[0]), bytes, hipMemcpyDeviceToHost,
+                           s) != hipSuccess) { st = SORT_STATUS_DEVICE_ERROR; break; }
+        if (values && hipMemcpyAsync(values, reinterpret_cast<void*>(d.values1[0]), bytes,
+                                     hipMemcpyDeviceToHost, s) != hipSuccess) {
+            st = SORT_STATUS_DEVICE_ERROR; break;
+        }
+        if (hipStreamSynchronize(s) != hipSuccess) st = SORT_STATUS_DEVICE_ERROR;
+    } while (0);
+    (void)hipStreamSynchronize(s);
+    sortFreeData(&d);
+    return st;
+}
+
+// sortDevice (mgpusort.cpp:782-806).  When the pass count is odd the result
+// is copied from keys[0] (the library buffer) back into the caller's buffer;
+// the reference copies from keys[1], which by then is the caller's own
+// unsorted buffer.
+sortStatus_t SORTAPI sortDevice(sortEngine_t engine, CUdeviceptr keys, CUdeviceptr values,
+                                int numElements, int numBits) {
+    if (!engine || !keys || numElements < 0 || numBits < 0 || numBits > 32)
+        return SORT_STATUS_INVALID_VALUE;
+    sortData_d d;
+    std::memset(&d, 0, sizeof(d));
+    d.keys[0] = keys;
+    if (values) d.values1[0] = values;
+    d.maxElements = numElements;
+    d.numElements = numElements;
+    d.valueCount = values ? 1 : 0;
+    sortStatus_t st = sortAllocData(engine, &d);
+    if (st) return st;
+    d.endBit = numBits;
+    st = sortArray(engine, &d);
+    size_t bytes = sizeof(uint32_t) * (size_t)numElements;
+    if (!st && d.parity) {
+        if (hipMemcpyAsync(reinterpret_cast<void*>(keys), reinterpret_cast<void*>(d.keys[0]), bytes,
+                           hipMemcpyDeviceToDevice, engine->stream) != hipSuccess)
+            st = SORT_STATUS_DEVICE_ERROR;
+        if (!st && values &&
+            hipMemcpyAsync(reinterpret_cast<void*>(values), reinterpret_cast<void*>(d.values1[0]),
+                           bytes, hipMemcpyDeviceToDevice, engine->stream) != hipSuccess)
+            st = SORT_STATUS_DEVICE_ERROR;
+    }
+    if (hipStreamSynchronize(engine->stream) != hipSuccess && !st) st = SORT_STATUS_DEVICE_ERROR;
+    // Free only what sortAllocData added: the caller's buffers sit at
+    // [parity] after the sort.
+    int p = d.parity & 1;
+    d.keys[p] = 0;
+    if (values) d.values1[p] = 0;
+    sortFreeData(&d);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Argument blocks of the radix-sort code objects.  Each kernel takes one of
+// these structs by value; the host engine fills the same struct and passes its
+// address as the single entry of hipModuleLaunchKernel's kernelParams, so this
+// header is the whole host/device contract.
+#pragma once
+#include <stdint.h>
+
+#define MGPU_RADIX_BITS 8
+#define MGPU_RADIX 256
+#define MGPU_MAX_PASSES 4
+#define MGPU_MAX_VALUES 6
+
+// Lookback word: bits 31..30 = state, bits 29..0 = a count.
+#define MGPU_LB_AGGREGATE 0x40000000u  // count = this tile's digit total
+#define MGPU_LB_PREFIX 0x80000000u     // count = inclusive prefix over tiles
+#define MGPU_LB_STATE 0xC0000000u
+#define MGPU_LB_VALUE 0x3FFFFFFFu
+
+// Value handling of one digit pass.
+#define MGPU_VALUES_NONE 0    // keys only
+#define MGPU_VALUES_INDEX 1   // write source indices to vals_out[0]
+#define MGPU_VALUES_SINGLE 2  // carry vals_in[0]
+#define MGPU_VALUES_MULTI 3   // carry vals_in[0 .. num_values)
+
+// digit_count: histograms of every pass's digit over all keys in one read.
+struct mgpuCountArgs {
+    const uint32_t* keys;
+    uint32_t* hist;          // [MGPU_MAX_PASSES][MGPU_RADIX], zero on entry
+    uint32_t* clear;         // words zeroed on the way (the lookback words)
+    uint64_t clear_words;    // multiple of 4
+    uint32_t n;
+    uint32_t chunk;          // keys per workgroup, multiple of 4
+    uint32_t num_passes;
+    uint32_t vec_ok;         // keys pointer is 16-byte aligned
+    uint32_t shift[MGPU_MAX_PASSES];
+    uint32_t mask[MGPU_MAX_PASSES];
+};
+
+// digit_scan: exclusive scan of each pass's histogram into digit starts; puts
+// the histogram and the tile counters back to zero for the next sort.
+struct mgpuScanArgs {
+    uint32_t* hist;           // [MGPU_MAX_PASSES][MGPU_RADIX] -> zeroed
+    uint32_t* digit_start;    // [MGPU_MAX_PASSES][MGPU_RADIX]
+    uint32_t* tile_counters;  // [MGPU_MAX_PASSES] -> zeroed
+    uint32_t num_passes;
+};
+
+// rank_scatter: one stable digit pass (rank in the tile, decoupled lookback
+// across tiles, LDS-staged scatter).
+struct mgpuPassArgs {
+    const uint32_t* keys_in;
+    uint32_t* keys_out;
+    const uint32_t* vals_in[MGPU_MAX_VALUES];
+    uint32_t* vals_out[MGPU_MAX_VALUES];
+    const uint32_t* digit_start;  // [MGPU_RADIX] for this pass
+    uint32_t* lookback;           // [num_tiles][MGPU_RADIX], zero on entry
+    uint32_t* tile_counter;       // zero on entry
+    uint32_t* error_flag;         // set to 1 if a lookback wait gave up
+    uint32_t n;
+    uint32_t num_tiles;
+    uint32_t shift;
+    uint32_t mask;
+    uint32_t num_values;          // MGPU_VALUES_MULTI only
+    uint32_t pad_;
+};

@@ -1,0 +1,80 @@
+// Code-object loading for libmgpusort: the role of LoadCountModule /
+// LoadHistModule / LoadSortModule (sort/src/mgpusort/mgpusort.cpp:44-177),
+// over hipModuleLoad / hipModuleGetFunction instead of cuModuleLoad.
+// Names are kernelPath + "<object>.hsaco" (prefix concatenation, as the
+// reference builds kernelPath + "count.cubin").
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "engine_internal.h"
+
+namespace {
+
+const char* const kRadixObject = "mgpusort_radix.hsaco";
+
+sortStatus_t get_fn(sortEngine_d* e, hipModule_t m, const std::string& name, hipFunction_t* f) {
+    if (hipModuleGetFunction(f, m, name.c_str()) != hipSuccess) return SORT_STATUS_KERNEL_ERROR;
+    e->kernelNames[*f] = name;
+    return SORT_STATUS_SUCCESS;
+}
+
+// Tile shape: MGPUSORT_TILE="<threads>x<items>" picks one of the built
+// shapes (for sweeps); otherwise the default.
+SortConfig pick_config() {
+    SortConfig c = {256, 16};
+    const char* t = std::getenv("MGPUSORT_TILE");
+    if (t && *t) {
+        unsigned b = 0, i = 0;
+        if (std::sscanf(t, "%ux%u", &b, &i) == 2 && b && i) c = {b, i};
+    }
+    return c;
+}
+
+}  // namespace
+
+sortStatus_t mgpu_engine_load(sortEngine_d* e) {
+    std::string path = e->kernelPath + kRadixObject;
+    if (hipModuleLoad(&e->radixModule, path.c_str()) != hipSuccess) {
+        e->radixModule = nullptr;
+        return SORT_STATUS_KERNEL_NOT_FOUND;
+    }
+    sortStatus_t st = get_fn(e, e->radixModule, "mgpu_digit_count", &e->fnCount);
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
+    e->config = pick_config();
+    for (int v = 0; v < 4 && !st; ++v) {
+        char name[96];
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_v%d", e->config.blockThreads,
+                      e->config.itemsPerThread, v);
+        st = get_fn(e, e->radixModule, name, &e->fnPass[v]);
+    }
+    if (st) return st;
+
+    const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_MAX_PASSES + 4;
+    if (hipMalloc(&e->fixedScratch, words * sizeof(uint32_t)) != hipSuccess) {
+        e->fixedScratch = nullptr;
+        return SORT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    if (hipMemset(e->fixedScratch, 0, words * sizeof(uint32_t)) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    e->hist = e->fixedScratch;
+    e->digitStart = e->hist + MGPU_MAX_PASSES * MGPU_RADIX;
+    e->tileCounters = e->digitStart + MGPU_MAX_PASSES * MGPU_RADIX;
+    e->errorFlag = e->tileCounters + MGPU_MAX_PASSES;
+    return SORT_STATUS_SUCCESS;
+}
+
+void mgpu_engine_destroy(sortEngine_d* e) {
+    if (!e) return;
+    if (e->stream || e->radixModule) (void)hipStreamSynchronize(e->stream);
+    for (auto& t : e->timed) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    for (hipEvent_t ev : e->eventPool) (void)hipEventDestroy(ev);
+    if (e->lookback) (void)hipFree(e->lookback);
+    if (e->fixedScratch) (void)hipFree(e->fixedScratch);
+    if (e->radixModule) (void)hipModuleUnload(e->radixModule);
+    delete e;
+}

@@ -1,0 +1,404 @@
+// LSD radix sort pass kernels for gfx950 (CDNA4, wave64).
+//
+// One sort = digit_count (one read of the keys: the digit histogram of every
+// pass) -> digit_scan (exclusive scan of those histograms) -> one
+// rank_scatter launch per digit pass.
+//
+// Reference kernels this replaces (seanbaxter/mgpu_old, sort/src/kernels):
+//   CountBuckets_n            count.cu:48-333      -> mgpu_digit_count
+//   HistogramReduce_{1,2,3}_n hist1.cu, hist2.cu,  -> mgpu_digit_scan plus the
+//                             hist3.cu             decoupled lookback inside
+//                                                  rank_scatter
+//   RadixSort_n (4 modes)     sort.cu:54-479       -> mgpu_rank_scatter_*
+// The reference counts per sort block and scans (block, digit) counts in three
+// launches every pass; here every pass's global digit totals come from one
+// read up front, and the per-tile offsets come from a lookback across tiles
+// inside the scatter kernel, so a pass reads its keys and values once and
+// writes them once.
+//
+// Stable ranking inside a tile (wave64): keys are loaded striped, lane l of
+// wave w holding key  tile_base + w*64*IPT + j*64 + l  in slot j, so the
+// order (wave, slot, lane) is the input order.  For each slot the wave ranks
+// its 64 keys against each other by digit with a match mask (the lanes that
+// hold the same digit), the rank being the number of matching lower lanes
+// plus the wave's running count of that digit.  The match mask is built
+// with LDS atomic ORs (default) or, with MGPU_RANK_BALLOT=1, with one ballot
+// per digit bit.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernel_args.h"
+
+#ifndef MGPU_RANK_BALLOT
+#define MGPU_RANK_BALLOT 0
+#endif
+
+namespace {
+
+constexpr uint32_t RADIX = MGPU_RADIX;
+// Polls of one lookback word before a pass gives up (about a second); only a
+// broken tile counter could make a wait that long.
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t ld_acquire_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inclusive scan over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x, uint32_t lane) {
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    return x;
+}
+
+// Exclusive scan of x over the first 256 threads of the block (threads past
+// 256 must pass 0).  s_tmp holds one word per wave.  Contains barriers: every
+// thread of the block must call it.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s_tmp,
+                                                         uint32_t lane, uint32_t wave) {
+    constexpr int W = BLOCK / 64;
+    uint32_t inc = wave_inclusive_scan(x, lane);
+    if (lane == 63) s_tmp[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w)
+        if ((uint32_t)w < wave) before += s_tmp[w];
+    __syncthreads();
+    return before + inc - x;
+}
+
+// ---------------------------------------------------------------------------
+// digit_count: per-wave LDS histograms of all passes, merged into the global
+// histogram with one atomic per (pass, digit) per workgroup.  Also zeroes the
+// lookback words the passes of this sort will use.
+template <int BLOCK>
+__device__ void digit_count_body(const mgpuCountArgs& a) {
+    constexpr int W = BLOCK / 64;
+    __shared__ uint32_t s_hist[W][MGPU_MAX_PASSES][RADIX];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    uint32_t* flat = &s_hist[0][0][0];
+    for (uint32_t i = tid; i < (uint32_t)(W * MGPU_MAX_PASSES * RADIX); i += BLOCK) flat[i] = 0;
+
+    // Zero this sort's lookback words (grid-stride, 16-byte stores).
+    {
+        uint4* c = reinterpret_cast<uint4*>(a.clear);
+        const uint64_t nvec = a.clear_words >> 2;
+        for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + tid; i < nvec;
+             i += (uint64_t)gridDim.x * BLOCK)
+            c[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+
+    const uint32_t np = a.num_passes;
+    uint32_t (*h)[RADIX] = s_hist[wave];
+    auto count_key = [&](uint32_t k) {
+#pragma unroll
+        for (int p = 0; p < MGPU_MAX_PASSES; ++p)
+            if ((uint32_t)p < np)
+                __hip_atomic_fetch_add(&h[p][(k >> a.shift[p]) & a.mask[p]], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+
+    const uint32_t begin = blockIdx.x * a.chunk;
+    const uint32_t end = min(a.n, begin + a.chunk);
+    if (begin < end) {
+        if (a.vec_ok) {
+            const uint4* v = reinterpret_cast<const uint4*>(a.keys + begin);
+            const uint32_t nvec = (end - begin) >> 2;
+            for (uint32_t i = tid; i < nvec; i += BLOCK) {
+                uint4 q = v[i];
+                count_key(q.x);
+                count_key(q.y);
+                count_key(q.z);
+                count_key(q.w);
+            }
+            for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) count_key(a.keys[i]);
+        } else {
+            for (uint32_t i = begin + tid; i < end; i += BLOCK) count_key(a.keys[i]);
+        }
+    }
+    __syncthreads();
+
+    for (uint32_t i = tid; i < (uint32_t)(MGPU_MAX_PASSES * RADIX); i += BLOCK) {
+        if (i / RADIX >= np) break;
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s += flat[w * MGPU_MAX_PASSES * RADIX + i];
+        if (s) __hip_atomic_fetch_add(&a.hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// digit_scan: one workgroup of 256 threads.
+__device__ void digit_scan_body(const mgpuScanArgs& a) {
+    __shared__ uint32_t s_tmp[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t p = 0; p < a.num_passes; ++p) {
+        uint32_t c = a.hist[p * RADIX + tid];
+        uint32_t e = block_exclusive_scan<256>(c, s_tmp, lane, wave);
+        a.digit_start[p * RADIX + tid] = e;
+        a.hist[p * RADIX + tid] = 0;
+    }
+    if (tid < MGPU_MAX_PASSES) a.tile_counters[tid] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// rank_scatter: one digit pass over one tile of BLOCK*IPT keys per workgroup.
+template <int BLOCK, int IPT, int VMODE>
+__device__ void rank_scatter_body(const mgpuPassArgs& a) {
+    static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
+    constexpr int W = BLOCK / 64;
+    constexpr uint32_t TILE = BLOCK * IPT;
+    constexpr uint32_t WAVE_TILE = 64 * IPT;
+
+    __shared__ uint32_t s_stage[TILE];
+    __shared__ uint32_t s_count[W][RADIX];
+#if !MGPU_RANK_BALLOT
+    __shared__ uint64_t s_match[W][RADIX];
+#endif
+    __shared__ uint32_t s_global[RADIX];
+    __shared__ uint32_t s_tmp[W];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63;
+    const uint32_t wave = tid >> 6;
+
+    if (tid == 0)
+        s_tile = __hip_atomic_fetch_add(a.tile_counter, 1u, __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
+        (&s_count[0][0])[i] = 0;
+#if !MGPU_RANK_BALLOT
+        (&s_match[0][0])[i] = 0;
+#endif
+    }
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    if (tile >= a.num_tiles) {  // only a counter left dirty could do this
+        if (tid == 0) st_agent(a.error_flag, 2u);
+        return;
+    }
+
+    const uint32_t n = a.n;
+    const uint32_t base = tile * TILE;
+    const uint32_t wave_base = base + wave * WAVE_TILE;
+    const uint32_t tile_valid = min(TILE, n - base);
+    const bool full = tile_valid == TILE;
+    const uint32_t shift = a.shift, mask = a.mask;
+
+    // Load keys (and the single value array) striped: coalesced 256-byte rows.
+    uint32_t key[IPT];
+    uint32_t val[IPT];
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) key[j] = a.keys_in[wave_base + j * 64 + lane];
+        if constexpr (VMODE == MGPU_VALUES_SINGLE) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) val[j] = a.vals_in[0][wave_base + j * 64 + lane];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            uint32_t idx = wave_base + j * 64 + lane;
+            key[j] = idx < n ? a.keys_in[idx] : 0xffffffffu;
+        }
+        if constexpr (VMODE == MGPU_VALUES_SINGLE) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                uint32_t idx = wave_base + j * 64 + lane;
+                val[j] = idx < n ? a.vals_in[0][idx] : 0u;
+            }
+        }
+    }
+
+    // Digit of slot j; keys past n get the top digit so that they rank after
+    // every real key of the tile (they are last in input order).
+    auto digit_of = [&](int j) -> uint32_t {
+        if (full) return (key[j] >> shift) & mask;
+        uint32_t idx = wave_base + j * 64 + lane;
+        return idx < n ? ((key[j] >> shift) & mask) : (RADIX - 1);
+    };
+
+    // Rank within the wave: rank[j] = earlier same-digit keys of this wave.
+    uint32_t rank[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const uint32_t d = digit_of(j);
+        uint64_t m;
+#if MGPU_RANK_BALLOT
+        m = ~0ull;
+#pragma unroll
+        for (int b = 0; b < MGPU_RADIX_BITS; ++b) {
+            const uint32_t bit = (d >> b) & 1u;
+            const uint64_t bb = __builtin_amdgcn_ballot_w64(bit != 0);
+            m &= bit ? bb : ~bb;
+        }
+#else
+        __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+        m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+#endif
+        const uint32_t below = lanes_below(m);
+        const uint32_t old = s_count[wave][d];
+        rank[j] = old + below;
+        if (below == 0) {  // lowest lane of the group advances the count
+            s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
+#if !MGPU_RANK_BALLOT
+            __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+#endif
+        }
+    }
+    __syncthreads();
+
+    // Per digit (thread d): wave-exclusive offsets, tile total, publish the
+    // tile total for the lookback.
+    uint32_t total = 0, tile_count = 0;
+    if (tid < RADIX) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            uint32_t c = s_count[w][tid];
+            s_count[w][tid] = sum;
+            sum += c;
+        }
+        total = sum;
+        tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
+        st_agent(&a.lookback[tile * RADIX + tid],
+                 (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
+    }
+    const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
+    if (tid < RADIX) {
+        uint32_t prefix = 0;
+        if (tile > 0) {
+            int32_t t = (int32_t)tile - 1;
+            uint32_t spins = 0;
+            while (true) {
+                uint32_t s = ld_acquire_agent(&a.lookback[(uint32_t)t * RADIX + tid]);
+                if (!(s & MGPU_LB_STATE)) {
+                    if (++spins > kSpinLimit) {
+                        st_agent(a.error_flag, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                prefix += s & MGPU_LB_VALUE;
+                if (s & MGPU_LB_PREFIX) break;
+                if (t == 0) {  // tile 0 always publishes a prefix
+                    st_agent(a.error_flag, 3u);
+                    break;
+                }
+                --t;
+            }
+            st_agent(&a.lookback[tile * RADIX + tid], MGPU_LB_PREFIX | (prefix + tile_count));
+        }
+        s_global[tid] = a.digit_start[tid] + prefix - local_start;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
+    }
+    __syncthreads();
+
+    // Tile-local position of every key; stage the keys in digit order.
+    uint32_t pos[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        pos[j] = s_count[wave][digit_of(j)] + rank[j];
+        s_stage[pos[j]] = key[j];
+    }
+    __syncthreads();
+
+    // Write out in tile order: consecutive threads hold consecutive positions,
+    // so each digit's run of the tile leaves as one contiguous stretch.
+    uint32_t dest[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+        const uint32_t p = i * BLOCK + tid;
+        dest[i] = n;
+        if (full || p < tile_valid) {
+            const uint32_t k = s_stage[p];
+            dest[i] = s_global[(k >> shift) & mask] + p;
+            // Offsets outside [0, n) can only come from a broken count or
+            // lookback: flag it instead of writing out of bounds.
+            if (dest[i] < n) a.keys_out[dest[i]] = k;
+            else st_agent(a.error_flag, 4u);
+        }
+    }
+
+    if constexpr (VMODE != MGPU_VALUES_NONE) {
+        const uint32_t nv = VMODE == MGPU_VALUES_MULTI ? a.num_values : 1u;
+        for (uint32_t v = 0; v < nv; ++v) {
+            if constexpr (VMODE == MGPU_VALUES_INDEX) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) val[j] = wave_base + j * 64 + lane;
+            } else if constexpr (VMODE == MGPU_VALUES_MULTI) {
+                const uint32_t* src = a.vals_in[v];
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    uint32_t idx = wave_base + j * 64 + lane;
+                    val[j] = (full || idx < n) ? src[idx] : 0u;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
+            __syncthreads();
+            uint32_t* dst = a.vals_out[v];
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) {
+                const uint32_t p = i * BLOCK + tid;
+                if ((full || p < tile_valid) && dest[i] < n) dst[dest[i]] = s_stage[p];
+            }
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Exported entry points (names looked up by the engine).
+
+extern "C" __global__ void __launch_bounds__(256) mgpu_digit_count(mgpuCountArgs a) {
+    digit_count_body<256>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(256) mgpu_digit_scan(mgpuScanArgs a) {
+    digit_scan_body(a);
+}
+
+#define MGPU_RANK_SCATTER(B, I)                                                               \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_v0(mgpuPassArgs a) {                                  \
+        rank_scatter_body<B, I, MGPU_VALUES_NONE>(a);                                         \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_v1(mgpuPassArgs a) {                                  \
+        rank_scatter_body<B, I, MGPU_VALUES_INDEX>(a);                                        \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_v2(mgpuPassArgs a) {                                  \
+        rank_scatter_body<B, I, MGPU_VALUES_SINGLE>(a);                                       \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_v3(mgpuPassArgs a) {                                  \
+        rank_scatter_body<B, I, MGPU_VALUES_MULTI>(a);                                        \
+    }
+
+MGPU_RANK_SCATTER(256, 16)
+MGPU_RANK_SCATTER(256, 24)
+MGPU_RANK_SCATTER(512, 16)

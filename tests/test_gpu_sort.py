@@ -1,0 +1,251 @@
+"""Parity of the HIP sort with the oracle, through the C ABI (needs a gfx950 GPU).
+
+Bit-exact: keys, every value array and generated indices must equal the
+oracle's stable sort by the bit field [firstBit, endBit) on the same seeded
+inputs.  Full-size workloads are checked by the size-independent properties
+that define the stable sort (check_stable_sort), smaller ones element by
+element against the oracle.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import mgpu_old_amd as mg
+from mgpu_old_amd import workloads
+from conftest import check_stable_sort
+from gpu_helpers import GpuSort, copy_d2d, make_engine, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine():
+    eng = make_engine()
+    yield eng
+    eng.close()
+
+
+def _check_vs_oracle(oracle, keys, g, values=(), value_count=None, fb=0, eb=32):
+    vc = len(values) if value_count is None else value_count
+    rk, rv = oracle.stable_sort(keys, list(values), value_count=vc, first_bit=fb, end_bit=eb)
+    out_k = g.keys()
+    assert np.array_equal(out_k, rk), "keys differ from the oracle"
+    for j, (o, r) in enumerate(zip(g.values(), rv)):
+        assert np.array_equal(o, r), f"value array {j} differs from the oracle"
+
+
+def test_config1_keys_only_2p20(engine, oracle):
+    """2^20 uniform u32 keys, [0, 32), keys only."""
+    keys = workloads.uniform_keys(1 << 20, seed=1)
+    g = GpuSort(keys).run(engine)
+    _check_vs_oracle(oracle, keys, g)
+    assert g.data.parity == 0
+
+
+def test_config2_key_value_2p25(engine, oracle):
+    """2^25 uniform keys + one value array holding 0..N-1."""
+    n = 1 << 25
+    keys = workloads.uniform_keys(n, seed=2)
+    vals = np.arange(n, dtype=np.uint32)
+    g = GpuSort(keys, [vals]).run(engine)
+    out_k, (out_v,) = g.keys(), g.values()
+    check_stable_sort(keys, out_k, out_v, 0, 32)
+    rk, (rv,) = oracle.stable_sort(keys, [vals])
+    assert np.array_equal(out_k, rk) and np.array_equal(out_v, rv)
+
+
+@pytest.mark.parametrize("mask_bits,end_bit", [(32, 32), (24, 24)])
+def test_config3_low_entropy_index_2p27(engine, mask_bits, end_bit):
+    """Low-entropy keys (AND of 3 words) with index generation; full-size
+    properties (the 2^28 run is in the benchmark's own parity check)."""
+    n = 1 << 27
+    keys = workloads.and3_keys(n, seed=3, mask_bits=mask_bits)
+    g = GpuSort(keys, value_count=-1, end_bit=end_bit).run(engine)
+    out_k, (idx,) = g.keys(), g.values()
+    check_stable_sort(keys, out_k, idx, 0, end_bit)
+    assert g.data.parity == len(mg.plan_passes(end_bit)) % 2
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 63, 64, 65, 2047, 2048, 2049, 4095, 4096, 4097, 65537,
+                               300_001])
+@pytest.mark.parametrize("vc", [-1, 0, 1, 2])
+def test_sizes_and_value_counts(engine, oracle, n, vc):
+    keys = workloads.uniform_keys(n, seed=n)
+    vals = [workloads.uniform_keys(n, seed=1000 + n + j) for j in range(max(vc, 0))]
+    g = GpuSort(keys, vals, value_count=vc).run(engine)
+    _check_vs_oracle(oracle, keys, g, vals, vc)
+
+
+@pytest.mark.parametrize("vc", [3, 4, 5, 6])
+def test_multi_value_arrays(engine, oracle, vc):
+    n = 123_457
+    keys = workloads.and3_keys(n, seed=vc)
+    vals = [workloads.uniform_keys(n, seed=77 + j) for j in range(vc)]
+    g = GpuSort(keys, vals).run(engine)
+    _check_vs_oracle(oracle, keys, g, vals)
+
+
+@pytest.mark.parametrize("fb,eb", [(0, 1), (31, 32), (7, 25), (0, 24), (4, 12), (3, 4), (0, 9),
+                                   (16, 32), (1, 31)])
+@pytest.mark.parametrize("vc", [0, 1, -1])
+def test_bit_ranges(engine, oracle, fb, eb, vc):
+    n = 70_001
+    keys = workloads.uniform_keys(n, seed=fb * 40 + eb)
+    vals = [np.arange(n, dtype=np.uint32)] if vc == 1 else []
+    g = GpuSort(keys, vals, value_count=vc, first_bit=fb, end_bit=eb).run(engine)
+    _check_vs_oracle(oracle, keys, g, vals, vc, fb, eb)
+    assert g.data.parity == len(mg.plan_passes(eb - fb)) % 2
+
+
+@pytest.mark.parametrize("kind", ["equal", "sorted", "reversed", "two_values", "skewed"])
+def test_adversarial_inputs(engine, oracle, kind):
+    n = 200_000
+    if kind == "equal":
+        keys = np.full(n, 0xDEADBEEF, dtype=np.uint32)
+    elif kind == "sorted":
+        keys = np.sort(workloads.uniform_keys(n, 5))
+    elif kind == "reversed":
+        keys = np.sort(workloads.uniform_keys(n, 6))[::-1].copy()
+    elif kind == "two_values":
+        keys = (workloads.uniform_keys(n, 7) & 1) * np.uint32(0xFFFFFFFF)
+    else:
+        keys = workloads.and3_keys(n, 8) & workloads.and3_keys(n, 9)
+    g = GpuSort(keys, value_count=-1).run(engine)
+    _check_vs_oracle(oracle, keys, g, value_count=-1)
+
+
+def test_zero_bits_and_zero_elements_do_nothing(engine):
+    keys = workloads.uniform_keys(5000, 11)
+    g = GpuSort(keys, value_count=-1, first_bit=9, end_bit=9).run(engine)
+    assert np.array_equal(g.keys(), keys) and g.data.parity == 0
+    g = GpuSort(np.zeros(0, dtype=np.uint32), value_count=1, capacity=16)
+    g.run(engine)
+    assert g.data.parity == 0
+
+
+@pytest.mark.parametrize("bit_pass", [1, 2, 3, 4, 5, 6, 0, 9])
+def test_sort_array_ex(engine, oracle, bit_pass):
+    n = 50_001
+    keys = workloads.uniform_keys(n, seed=bit_pass + 50)
+    fb, eb = 3, 27
+    g = GpuSort(keys, value_count=-1, first_bit=fb, end_bit=eb)
+    g.run(engine, ex=(256, 8, bit_pass, False))
+    _check_vs_oracle(oracle, keys, g, value_count=-1, fb=fb, eb=eb)
+    bp = 6 if bit_pass <= 0 or bit_pass > 6 else bit_pass
+    passes = -(-(eb - fb) // bp)
+    assert g.data.parity == passes % 2
+    # The array at [0] before the sort is at [parity] after it.
+    assert g.data.keys[g.data.parity] == g.start_keys
+
+
+def test_sort_array_ex_rejects_reference_invalid_shapes(engine):
+    g = GpuSort(workloads.uniform_keys(100, 1))
+    for args in [(64, 8, 6, False), (256, 4, 6, False), (128, 16, 6, False)]:
+        with pytest.raises(mg.MgpuError) as e:
+            g.run(engine, ex=args)
+        assert e.value.status == mg.SortStatus.INVALID_VALUE
+
+
+def test_invalid_descriptors(engine):
+    L = mg.lib()
+    g = GpuSort(workloads.uniform_keys(100, 1))
+    d = g.data
+    d.numElements = d.maxElements + 1
+    assert L.sortArray(engine.handle, ctypes.byref(d)) == mg.SortStatus.INVALID_VALUE
+    d.numElements = 100
+    d.endBit = 33
+    assert L.sortArray(engine.handle, ctypes.byref(d)) == mg.SortStatus.INVALID_VALUE
+    d.endBit, d.firstBit = 8, 9
+    assert L.sortArray(engine.handle, ctypes.byref(d)) == mg.SortStatus.INVALID_VALUE
+    d.firstBit, d.endBit, d.valueCount = 0, 32, 7
+    assert L.sortArray(engine.handle, ctypes.byref(d)) == mg.SortStatus.INVALID_VALUE
+    d.valueCount = 1  # value pointers missing
+    assert L.sortArray(engine.handle, ctypes.byref(d)) == mg.SortStatus.INVALID_VALUE
+
+
+def test_library_allocated_data_and_parity_chain(engine, oracle):
+    """sortCreateData buffers; two sorts in a row continue the parity chain
+    the way a multi-key client (bwt/src/mgpubwt/mgpubwt.cpp:262-292) uses it."""
+    torch = pytest.importorskip("torch")
+    L = mg.lib()
+    n = 9000
+    pd = ctypes.POINTER(mg.SortData)()
+    assert L.sortCreateData(engine.handle, n, 1, ctypes.byref(pd)) == 0
+    d = pd.contents
+    assert d.maxElements == 10240 and d.numElements == n and d.endBit == 32
+    assert all(d.keys[i] for i in range(2)) and all(d.values[0][i] for i in range(2))
+    assert not d.values[1][0]
+    lo = workloads.uniform_keys(n, 1)
+    k0 = d.keys[0]
+    torch.cuda.synchronize()
+    copy_d2d(k0, to_dev(lo).data_ptr(), 4 * n)
+    copy_d2d(d.values[0][0], to_dev(np.arange(n, dtype=np.uint32)).data_ptr(), 4 * n)
+    d.firstBit, d.endBit = 0, 24  # 3 passes: odd
+    assert L.sortArray(engine.handle, pd) == 0
+    torch.cuda.synchronize()
+    assert d.parity == 1 and d.keys[1] == k0
+    out = torch.empty(n, dtype=torch.int32, device="cuda")
+    copy_d2d(out.data_ptr(), d.keys[0], 4 * n)
+    rk, _ = oracle.stable_sort(lo, first_bit=0, end_bit=24)
+    assert np.array_equal(to_host(out, n), rk)
+    # Second sort (16 bits = 2 passes, even) continues from keys[0] and keeps
+    # the chain: the original buffer stays at [parity].
+    d.firstBit, d.endBit = 8, 24
+    assert L.sortArray(engine.handle, pd) == 0
+    torch.cuda.synchronize()
+    assert d.parity == 1 and d.keys[1] == k0
+    copy_d2d(out.data_ptr(), d.keys[0], 4 * n)
+    rk2, _ = oracle.stable_sort(rk, first_bit=8, end_bit=24)
+    assert np.array_equal(to_host(out, n), rk2)
+    assert L.sortDestroyData(pd) == 0
+
+
+def test_sort_host_and_device(engine, oracle):
+    L = mg.lib()
+    n = 33_333
+    keys = workloads.uniform_keys(n, 31)
+    vals = np.arange(n, dtype=np.uint32)
+    hk, hv = keys.copy(), vals.copy()
+    P = ctypes.POINTER(ctypes.c_uint32)
+    assert L.sortHost(engine.handle, hk.ctypes.data_as(P), hv.ctypes.data_as(P), n, 24) == 0
+    rk, (rv,) = oracle.stable_sort(keys, [vals], first_bit=0, end_bit=24)
+    assert np.array_equal(hk, rk) and np.array_equal(hv, rv)
+    # sortDevice with an odd pass count must still leave the result in the
+    # caller's buffers.
+    dk, dv = to_dev(keys), to_dev(vals)
+    import torch
+    torch.cuda.synchronize()
+    assert L.sortDevice(engine.handle, dk.data_ptr(), dv.data_ptr(), n, 24) == 0
+    assert np.array_equal(to_host(dk, n), rk) and np.array_equal(to_host(dv, n), rv)
+    dk = to_dev(keys)
+    torch.cuda.synchronize()
+    assert L.sortDevice(engine.handle, dk.data_ptr(), 0, n, 32) == 0
+    rk32, _ = oracle.stable_sort(keys)
+    assert np.array_equal(to_host(dk, n), rk32)
+
+
+def test_keys_past_the_end_are_untouched(engine):
+    n, cap = 5000, 8192
+    keys = workloads.uniform_keys(n, 4)
+    tail = workloads.uniform_keys(cap - n, 5)
+    g = GpuSort(keys, value_count=-1, capacity=cap)
+    g.tensors[0][n:].copy_(to_dev(tail)[: cap - n])
+    g.tensors[1][n:].fill_(7)
+    g.run(engine)
+    # Both key buffers keep their contents past numElements.
+    assert np.array_equal(to_host(g.tensors[0], cap)[n:], tail)
+    assert (to_host(g.tensors[1], cap)[n:] == 7).all()
+
+
+def test_repeat_determinism(engine):
+    n = 1_000_003
+    keys = workloads.and3_keys(n, 12)
+    first = None
+    for _ in range(10):
+        g = GpuSort(keys, value_count=-1).run(engine)
+        out = (g.keys(), g.values()[0])
+        if first is None:
+            first = out
+        else:
+            assert np.array_equal(out[0], first[0]) and np.array_equal(out[1], first[1])
