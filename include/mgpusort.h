@@ -193,6 +193,49 @@ sortStatus_t SORTAPI sortGetStream(sortEngine_t engine, void** stream);
  * their widths (widths may be NULL).  Host-only; needs no device. */
 int SORTAPI sortPlanPasses(int numBits, int* widths);
 
+/* ---- Segmented sort (no counterpart in the reference). ---------------------
+ * Many independent arrays sorted in one persistent launch.  Each segment ends
+ * up exactly as sortArray would leave it if called on that segment alone with
+ * the same bit range and value count: stable by the bit field, values
+ * following, the result on side (sortPlanPasses(endBit - firstBit) & 1) of its
+ * buffers.  Segments of up to 2^18 elements are sorted by one wave or one
+ * workgroup each (digit width and tile shape per size bucket from the tuning
+ * table, kernelPath + "mgpusort_seg_tuning.txt"); larger ones run the
+ * device-wide passes. */
+
+/* Sorts the segments [segmentStarts[i], segmentStarts[i+1]) of data's
+ * buffers (host array of numSegments + 1 nondecreasing offsets, the last at
+ * most numElements) by data's bit range and value count, then swaps buffers
+ * and toggles parity as sortArray does.  Blocks until done. */
+sortStatus_t SORTAPI sortSegmentedArray(sortEngine_t engine, sortData_t data, int numSegments,
+                                        const int* segmentStarts);
+
+/* One independent array of a batch with buffers of its own. */
+typedef struct {
+    CUdeviceptr keys[2];
+    CUdeviceptr values[6][2];
+    int numElements;
+} sortSegment_d;
+
+/* Sorts every segment of the batch (same bit range and value count for
+ * all); each result is left on side (sortPlanPasses(endBit - firstBit) & 1)
+ * of that segment's buffers.  Blocks until done. */
+sortStatus_t SORTAPI sortSegmentedBatch(sortEngine_t engine, const sortSegment_d* segments,
+                                        int numSegments, int valueCount, int firstBit,
+                                        int endBit);
+
+/* Reusable form of sortSegmentedArray: the work lists are built once and the
+ * launches captured in a hipGraph; each execute replays it on the engine's
+ * stream (asynchronously) and swaps data's buffers as sortArray does.  data
+ * must hold the buffers given at creation, in either order. */
+struct sortSegPlan_d;
+typedef struct sortSegPlan_d* sortSegPlan_t;
+sortStatus_t SORTAPI sortSegmentedPlanCreate(sortEngine_t engine, const sortData_d* data,
+                                             int numSegments, const int* segmentStarts,
+                                             sortSegPlan_t* plan);
+sortStatus_t SORTAPI sortSegmentedPlanExecute(sortSegPlan_t plan, sortData_t data);
+sortStatus_t SORTAPI sortSegmentedPlanDestroy(sortSegPlan_t plan);
+
 /* Waits for the engine's stream and reports SORT_STATUS_KERNEL_ERROR if a
  * kernel detected an inconsistency (a scatter offset out of range, or a
  * lookback wait that gave up) since the last call; SUCCESS otherwise. */

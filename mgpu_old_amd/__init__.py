@@ -11,7 +11,7 @@ import os
 
 __all__ = [
     "SortStatus", "MgpuError", "SortData", "lib", "library_path", "kernel_path",
-    "Engine", "MgpuSortData", "plan_passes",
+    "Engine", "MgpuSortData", "plan_passes", "SortSegment", "SegmentedPlan",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -65,6 +65,15 @@ class SortData(ctypes.Structure):
     ]
 
 
+class SortSegment(ctypes.Structure):
+    """sortSegment_d: one array of a segmented batch with its own buffers."""
+    _fields_ = [
+        ("keys", ctypes.c_uint64 * 2),
+        ("values", (ctypes.c_uint64 * 2) * 6),
+        ("numElements", ctypes.c_int),
+    ]
+
+
 class KernelTiming(ctypes.Structure):
     """sortKernelTiming_t: per-kernel aggregate of event-timed launches."""
     _fields_ = [
@@ -112,6 +121,13 @@ def lib():
             "sortSetStream": (ctypes.c_int, [eng, ctypes.c_void_p]),
             "sortGetStream": (ctypes.c_int, [eng, P(ctypes.c_void_p)]),
             "sortPlanPasses": (ctypes.c_int, [ctypes.c_int, P(ctypes.c_int)]),
+            "sortSegmentedArray": (ctypes.c_int, [eng, sd, ctypes.c_int, P(ctypes.c_int)]),
+            "sortSegmentedBatch": (ctypes.c_int, [eng, P(SortSegment), ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+            "sortSegmentedPlanCreate": (ctypes.c_int, [eng, sd, ctypes.c_int, P(ctypes.c_int),
+                                                       P(ctypes.c_void_p)]),
+            "sortSegmentedPlanExecute": (ctypes.c_int, [ctypes.c_void_p, sd]),
+            "sortSegmentedPlanDestroy": (ctypes.c_int, [ctypes.c_void_p]),
             "sortCheckDevice": (ctypes.c_int, [eng]),
             "sortEnableTiming": (ctypes.c_int, [eng, ctypes.c_bool]),
             "sortReadTiming": (ctypes.c_int, [eng, P(KernelTiming), ctypes.c_int]),
@@ -201,10 +217,53 @@ class Engine:
     def sort_array(self, data):
         _check(lib().sortArray(self._h, ctypes.byref(data)), "sortArray")
 
+    def sort_segmented(self, data, starts):
+        """sortSegmentedArray: starts = numSegments + 1 offsets into data."""
+        arr = (ctypes.c_int * len(starts))(*[int(x) for x in starts])
+        _check(lib().sortSegmentedArray(self._h, ctypes.byref(data), len(starts) - 1, arr),
+               "sortSegmentedArray")
+
+    def sort_batch(self, segments, value_count, first_bit=0, end_bit=32):
+        """sortSegmentedBatch over a list of SortSegment."""
+        arr = (SortSegment * max(1, len(segments)))(*segments)
+        _check(lib().sortSegmentedBatch(self._h, arr, len(segments), value_count, first_bit,
+                                        end_bit), "sortSegmentedBatch")
+
+    def segmented_plan(self, data, starts):
+        return SegmentedPlan(self, data, starts)
+
     def sort_array_ex(self, data, num_threads=256, values_per_thread=8, bit_pass=6,
                       use_trans_list=False):
         _check(lib().sortArrayEx(self._h, ctypes.byref(data), num_threads, values_per_thread,
                                  bit_pass, use_trans_list), "sortArrayEx")
+
+
+class SegmentedPlan:
+    """sortSegmentedPlanCreate / Execute / Destroy: a graph-captured segmented
+    sort replayed on the engine's stream."""
+
+    def __init__(self, engine, data, starts):
+        self._engine = engine
+        self._h = ctypes.c_void_p()
+        arr = (ctypes.c_int * len(starts))(*[int(x) for x in starts])
+        _check(lib().sortSegmentedPlanCreate(engine.handle, ctypes.byref(data), len(starts) - 1,
+                                             arr, ctypes.byref(self._h)),
+               "sortSegmentedPlanCreate")
+
+    def execute(self, data):
+        _check(lib().sortSegmentedPlanExecute(self._h, ctypes.byref(data)),
+               "sortSegmentedPlanExecute")
+
+    def close(self):
+        if self._h:
+            lib().sortSegmentedPlanDestroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class MgpuSortData(SortData):

@@ -22,8 +22,9 @@ DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 CODE_OBJECTS = {
     "mgpusort_radix.hsaco": ["radixsort.hip"],
+    "mgpusort_seg.hsaco": ["segsort.hip"],
 }
-HOST_SOURCES = ["engine.cpp", "loader.cpp"]
+HOST_SOURCES = ["engine.cpp", "loader.cpp", "segmented.cpp"]
 
 
 def _run(cmd):

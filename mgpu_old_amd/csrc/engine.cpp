@@ -52,24 +52,53 @@ std::string library_dir() {
 }
 
 // Scratch that grows only (AllocSortResources, mgpusort.cpp:319-343).
-sortStatus_t ensure_lookback(sortEngine_d* e, uint64_t words) {
-    if (words <= e->lookbackWords) return SORT_STATUS_SUCCESS;
-    if (e->lookback) {
-        if (hipStreamSynchronize(e->stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
-        (void)hipFree(e->lookback);
-        e->lookback = nullptr;
-        e->lookbackWords = 0;
+sortStatus_t ensure_lookback(SortLane& L, uint64_t words) {
+    if (words <= L.lookbackWords) return SORT_STATUS_SUCCESS;
+    if (L.lookback) {
+        if (hipStreamSynchronize(L.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+        (void)hipFree(L.lookback);
+        L.lookback = nullptr;
+        L.lookbackWords = 0;
     }
     uint64_t alloc = words + (words >> 3) + 1024;  // headroom against regrowth
-    if (hipMalloc(&e->lookback, alloc * sizeof(uint32_t)) != hipSuccess) {
-        e->lookback = nullptr;
+    if (hipMalloc(&L.lookback, alloc * sizeof(uint32_t)) != hipSuccess) {
+        L.lookback = nullptr;
         return SORT_STATUS_DEVICE_ALLOC_FAILED;
     }
-    e->lookbackWords = alloc & ~uint64_t(3);
+    L.lookbackWords = alloc & ~uint64_t(3);
     return SORT_STATUS_SUCCESS;
 }
 
 }  // namespace
+
+sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes) {
+    const uint32_t tile = e->config.blockThreads * e->config.itemsPerThread;
+    const uint64_t tiles = (n + tile - 1) / tile;
+    return ensure_lookback(lane, (uint64_t)passes * tiles * MGPU_RADIX);
+}
+
+sortStatus_t mgpu_lane_init(SortLane* l) {
+    const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_MAX_PASSES;
+    if (hipMalloc(&l->fixed, words * sizeof(uint32_t)) != hipSuccess) {
+        l->fixed = nullptr;
+        return SORT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    if (hipMemset(l->fixed, 0, words * sizeof(uint32_t)) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    l->hist = l->fixed;
+    l->digitStart = l->hist + MGPU_MAX_PASSES * MGPU_RADIX;
+    l->tileCounters = l->digitStart + MGPU_MAX_PASSES * MGPU_RADIX;
+    l->histDirty = false;
+    return SORT_STATUS_SUCCESS;
+}
+
+void mgpu_lane_free(SortLane* l) {
+    if (l->lookback) (void)hipFree(l->lookback);
+    if (l->fixed) (void)hipFree(l->fixed);
+    l->lookback = nullptr;
+    l->lookbackWords = 0;
+    l->fixed = nullptr;
+}
 
 // ---------------------------------------------------------------------------
 // Internal helpers shared with the segmented path.
@@ -101,22 +130,20 @@ hipEvent_t pooled_event(sortEngine_d* e) {
 
 }  // namespace
 
-sortStatus_t mgpu_launch(sortEngine_d* e, hipFunction_t f, unsigned grid, unsigned block,
-                         void* args, size_t argBytes) {
-    (void)argBytes;
+sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, unsigned grid,
+                         unsigned block, void* args) {
     void* params[] = {args};
     hipEvent_t start = nullptr, stop = nullptr;
     if (e->timing) {
         start = pooled_event(e);
         stop = pooled_event(e);
-        if (!start || !stop || hipEventRecord(start, e->stream) != hipSuccess)
+        if (!start || !stop || hipEventRecord(start, stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
     }
-    hipError_t r = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, e->stream, params,
-                                         nullptr);
+    hipError_t r = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr);
     if (r != hipSuccess) return SORT_STATUS_LAUNCH_ERROR;
     if (e->timing) {
-        if (hipEventRecord(stop, e->stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+        if (hipEventRecord(stop, stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
         e->timed.push_back({f, start, stop});
     }
     return SORT_STATUS_SUCCESS;
@@ -125,8 +152,9 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipFunction_t f, unsigned grid, unsign
 // One sort of n keys by [firstBit, firstBit + sum(widths)) from in-buffers to
 // out-buffers alternating per pass: pass p reads keys[p&1]-side pointers.
 // Pointers are passed per pass by the caller through `bufs`.
-sortStatus_t mgpu_sort_passes(sortEngine_d* e, uint32_t n, int firstBit, int numPasses,
-                              const int* widths, int valueCount, uint32_t* const keyBuf[2],
+sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firstBit,
+                              int numPasses, const int* widths, int valueCount,
+                              uint32_t* const keyBuf[2],
                               uint32_t* const valBuf[MGPU_MAX_VALUES][2]) {
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     if (numPasses > MGPU_MAX_PASSES) return SORT_STATUS_INTERNAL_ERROR;
@@ -134,24 +162,24 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, uint32_t n, int firstBit, int num
     const uint32_t tile = cfg.blockThreads * cfg.itemsPerThread;
     const uint32_t numTiles = (n + tile - 1) / tile;
     const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
-    sortStatus_t st = ensure_lookback(e, lbWords);
+    sortStatus_t st = ensure_lookback(L, lbWords);
     if (st) return st;
 
-    if (e->histDirty) {
-        if (hipMemsetAsync(e->hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
-                           e->stream) != hipSuccess ||
-            hipMemsetAsync(e->tileCounters, 0, MGPU_MAX_PASSES * sizeof(uint32_t), e->stream) !=
+    if (L.histDirty) {
+        if (hipMemsetAsync(L.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
+                           L.stream) != hipSuccess ||
+            hipMemsetAsync(L.tileCounters, 0, MGPU_MAX_PASSES * sizeof(uint32_t), L.stream) !=
                 hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
-        e->histDirty = false;
+        L.histDirty = false;
     }
 
     // digit_count
     mgpuCountArgs ca;
     std::memset(&ca, 0, sizeof(ca));
     ca.keys = keyBuf[0];
-    ca.hist = e->hist;
-    ca.clear = e->lookback;
+    ca.hist = L.hist;
+    ca.clear = L.lookback;
     ca.clear_words = (lbWords + 3) & ~uint64_t(3);
     ca.n = n;
     ca.num_passes = (uint32_t)numPasses;
@@ -175,19 +203,19 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, uint32_t n, int firstBit, int num
     uint32_t clearGrid = (uint32_t)((ca.clear_words / 4 + 256 * 16 - 1) / (256 * 16));
     if (clearGrid > maxGrid) clearGrid = maxGrid;
     if (clearGrid > grid) grid = clearGrid;
-    e->histDirty = true;
-    st = mgpu_launch(e, e->fnCount, grid, 256, &ca, sizeof(ca));
+    L.histDirty = true;
+    st = mgpu_launch(e, L.stream, e->fnCount, grid, 256, &ca);
     if (st) return st;
 
     // digit_scan
     mgpuScanArgs sa;
-    sa.hist = e->hist;
-    sa.digit_start = e->digitStart;
-    sa.tile_counters = e->tileCounters;
+    sa.hist = L.hist;
+    sa.digit_start = L.digitStart;
+    sa.tile_counters = L.tileCounters;
     sa.num_passes = (uint32_t)numPasses;
-    st = mgpu_launch(e, e->fnScan, 1, 256, &sa, sizeof(sa));
+    st = mgpu_launch(e, L.stream, e->fnScan, 1, 256, &sa);
     if (st) return st;
-    e->histDirty = false;
+    L.histDirty = false;
 
     // rank_scatter per pass
     int bit = firstBit;
@@ -207,16 +235,16 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, uint32_t n, int firstBit, int num
             pa.vals_in[v] = valBuf[v][in];
             pa.vals_out[v] = valBuf[v][out];
         }
-        pa.digit_start = e->digitStart + p * MGPU_RADIX;
-        pa.lookback = e->lookback + (uint64_t)p * numTiles * MGPU_RADIX;
-        pa.tile_counter = e->tileCounters + p;
+        pa.digit_start = L.digitStart + p * MGPU_RADIX;
+        pa.lookback = L.lookback + (uint64_t)p * numTiles * MGPU_RADIX;
+        pa.tile_counter = L.tileCounters + p;
         pa.error_flag = e->errorFlag;
         pa.n = n;
         pa.num_tiles = numTiles;
         pa.shift = (uint32_t)bit;
         pa.mask = (1u << widths[p]) - 1u;
         pa.num_values = (uint32_t)carried;
-        st = mgpu_launch(e, e->fnPass[mode], numTiles, cfg.blockThreads, &pa, sizeof(pa));
+        st = mgpu_launch(e, L.stream, e->fnPass[mode], numTiles, cfg.blockThreads, &pa);
         if (st) return st;
         bit += widths[p];
     }
@@ -259,8 +287,8 @@ sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, co
     for (int v = 0; v < MGPU_MAX_VALUES; ++v)
         for (int s = 0; s < 2; ++s) valBuf[v][s] = reinterpret_cast<uint32_t*>(data->values[v][s]);
 
-    sortStatus_t st = mgpu_sort_passes(e, n, data->firstBit, numPasses, widths, data->valueCount,
-                                       keyBuf, valBuf);
+    sortStatus_t st = mgpu_sort_passes(e, e->lane, n, data->firstBit, numPasses, widths,
+                                       data->valueCount, keyBuf, valBuf);
     if (st) return st;
     if (numPasses & 1) {
         std::swap(data->keys[0], data->keys[1]);
@@ -343,13 +371,13 @@ sortStatus_t SORTAPI sortDestroyEngine(sortEngine_t engine) { return sortRelease
 
 sortStatus_t SORTAPI sortSetStream(sortEngine_t engine, void* stream) {
     if (!engine) return SORT_STATUS_INVALID_VALUE;
-    engine->stream = reinterpret_cast<hipStream_t>(stream);
+    engine->lane.stream = reinterpret_cast<hipStream_t>(stream);
     return SORT_STATUS_SUCCESS;
 }
 
 sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine) {
     if (!engine) return SORT_STATUS_INVALID_VALUE;
-    if (hipStreamSynchronize(engine->stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    if (hipStreamSynchronize(engine->lane.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
     uint32_t flag = 0;
     if (hipMemcpy(&flag, engine->errorFlag, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
         return SORT_STATUS_DEVICE_ERROR;
@@ -367,7 +395,7 @@ sortStatus_t SORTAPI sortEnableTiming(sortEngine_t engine, bool enable) {
 
 int SORTAPI sortReadTiming(sortEngine_t engine, sortKernelTiming_t* out, int maxEntries) {
     if (!engine) return -1;
-    if (hipStreamSynchronize(engine->stream) != hipSuccess) return -1;
+    if (hipStreamSynchronize(engine->lane.stream) != hipSuccess) return -1;
     std::vector<sortKernelTiming_t> agg;
     std::map<hipFunction_t, size_t> slot;
     for (auto& t : engine->timed) {
@@ -397,7 +425,7 @@ int SORTAPI sortReadTiming(sortEngine_t engine, sortKernelTiming_t* out, int max
 
 sortStatus_t SORTAPI sortGetStream(sortEngine_t engine, void** stream) {
     if (!engine || !stream) return SORT_STATUS_INVALID_VALUE;
-    *stream = reinterpret_cast<void*>(engine->stream);
+    *stream = reinterpret_cast<void*>(engine->lane.stream);
     return SORT_STATUS_SUCCESS;
 }
 
@@ -535,7 +563,7 @@ sortStatus_t SORTAPI sortHost(sortEngine_t engine, unsigned int* keys, unsigned 
     if (st) return st;
     d.endBit = numBits;
     size_t bytes = sizeof(uint32_t) * (size_t)numElements;
-    hipStream_t s = engine->stream;
+    hipStream_t s = engine->lane.stream;
     do {
         if (hipMemcpyAsync(reinterpret_cast<void*>(d.keys[0]), keys, bytes, hipMemcpyHostToDevice,
                            s) != hipSuccess) { st = SORT_STATUS_DEVICE_ERROR; break; }
@@ -580,14 +608,14 @@ sortStatus_t SORTAPI sortDevice(sortEngine_t engine, CUdeviceptr keys, CUdevicep
     size_t bytes = sizeof(uint32_t) * (size_t)numElements;
     if (!st && d.parity) {
         if (hipMemcpyAsync(reinterpret_cast<void*>(keys), reinterpret_cast<void*>(d.keys[0]), bytes,
-                           hipMemcpyDeviceToDevice, engine->stream) != hipSuccess)
+                           hipMemcpyDeviceToDevice, engine->lane.stream) != hipSuccess)
             st = SORT_STATUS_DEVICE_ERROR;
         if (!st && values &&
             hipMemcpyAsync(reinterpret_cast<void*>(values), reinterpret_cast<void*>(d.values1[0]),
-                           bytes, hipMemcpyDeviceToDevice, engine->stream) != hipSuccess)
+                           bytes, hipMemcpyDeviceToDevice, engine->lane.stream) != hipSuccess)
             st = SORT_STATUS_DEVICE_ERROR;
     }
-    if (hipStreamSynchronize(engine->stream) != hipSuccess && !st) st = SORT_STATUS_DEVICE_ERROR;
+    if (hipStreamSynchronize(engine->lane.stream) != hipSuccess && !st) st = SORT_STATUS_DEVICE_ERROR;
     // Free only what sortAllocData added: the caller's buffers sit at
     // [parity] after the sort.
     int p = d.parity & 1;

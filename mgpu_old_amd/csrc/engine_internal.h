@@ -10,6 +10,7 @@
 
 #include "../../include/mgpusort.h"
 #include "kernel_args.h"
+#include "segmented_internal.h"
 
 // Tile shape of the rank_scatter kernel: blockThreads * itemsPerThread keys.
 struct SortConfig {
@@ -17,12 +18,29 @@ struct SortConfig {
     uint32_t itemsPerThread;
 };
 
+// One in-order queue of sorts: a stream plus the device scratch of the
+// device-wide passes.  The engine owns one (its default lane, on the stream
+// chosen with sortSetStream); the sort server owns one per in-flight batch.
+struct SortLane {
+    hipStream_t stream = nullptr;
+    uint32_t* fixed = nullptr;  // hist[4][256], digitStart[4][256], tileCounters[4]
+    uint32_t* hist = nullptr;
+    uint32_t* digitStart = nullptr;
+    uint32_t* tileCounters = nullptr;
+    bool histDirty = false;  // hist/tileCounters may hold a partial sort's counts
+    uint32_t* lookback = nullptr;  // grow-only
+    uint64_t lookbackWords = 0;
+};
+
+sortStatus_t mgpu_lane_init(SortLane* l);
+void mgpu_lane_free(SortLane* l);
+
 struct sortEngine_d {
     int refcount = 1;  // sortIncEngine / sortReleaseEngine (not atomic, as util/util.h:84-98)
     int device = -1;
     int numCUs = 0;
     std::string kernelPath;
-    hipStream_t stream = nullptr;
+    SortLane lane;  // default lane: sortArray and friends run here
 
     hipModule_t radixModule = nullptr;
     hipFunction_t fnCount = nullptr;
@@ -30,18 +48,15 @@ struct sortEngine_d {
     hipFunction_t fnPass[4] = {};  // by value mode (MGPU_VALUES_*)
     SortConfig config = {256, 16};
 
-    // Fixed scratch: hist[4][256], digitStart[4][256], tileCounters[4],
-    // errorFlag -- one allocation.
-    uint32_t* fixedScratch = nullptr;
-    uint32_t* hist = nullptr;
-    uint32_t* digitStart = nullptr;
-    uint32_t* tileCounters = nullptr;
+    // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;
-    bool histDirty = false;  // hist/tileCounters may hold a partial sort's counts
 
-    // Lookback words (grow-only).
-    uint32_t* lookback = nullptr;
-    uint64_t lookbackWords = 0;
+    // Segmented sort: kernel variants [radix bits - 4][items 16?] and the
+    // size-bucket tuning table.
+    hipModule_t segModule = nullptr;
+    hipFunction_t segFn[kSegMaxBits - kSegMinBits + 1][2] = {};
+    int segOccupancy[kSegMaxBits - kSegMinBits + 1][2] = {};
+    std::vector<SegTuneEntry> segTable;
 
     // Per-kernel timing (sortEnableTiming / sortReadTiming).
     struct TimedLaunch {
@@ -63,12 +78,17 @@ void mgpu_engine_destroy(sortEngine_d* e);
 // pass count and fills widths (narrower passes first).
 int mgpu_plan_passes(int numBits, int maxWidth, int* widths);
 
-sortStatus_t mgpu_launch(sortEngine_d* e, hipFunction_t f, unsigned grid, unsigned block,
-                         void* args, size_t argBytes);
+sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, unsigned grid,
+                         unsigned block, void* args);
+
+// Grows the lane's lookback words to cover a device-wide sort of n keys in
+// `passes` passes (a reallocation synchronizes the lane's stream).
+sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes);
 
 // Runs numPasses digit passes over n keys starting at firstBit.  Pass p reads
 // buffer side (p & 1) and writes side (p & 1) ^ 1 of keyBuf / valBuf, so the
 // result is on side (numPasses & 1).  valueCount as in sortData_d.
-sortStatus_t mgpu_sort_passes(sortEngine_d* e, uint32_t n, int firstBit, int numPasses,
+sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& lane, uint32_t n, int firstBit,
+                              int numPasses,
                               const int* widths, int valueCount, uint32_t* const keyBuf[2],
                               uint32_t* const valBuf[MGPU_MAX_VALUES][2]);

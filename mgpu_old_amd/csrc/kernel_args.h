@@ -45,6 +45,38 @@ struct mgpuScanArgs {
     uint32_t num_passes;
 };
 
+// ---------------------------------------------------------------------------
+// Segmented sort: many independent arrays in one persistent launch.
+
+#define MGPU_SEG_MAX_PASSES 8
+
+// One segment: its key and value buffers on both sides of the ping-pong.
+// Side 0 holds the input; the result goes to side `result_side` of the
+// launch (the side sortArray would leave it on).
+struct mgpuSegDesc {
+    uint32_t* keys[2];
+    uint32_t* vals[MGPU_MAX_VALUES][2];
+    uint32_t n;
+    uint32_t pad_;
+};
+
+struct mgpuSegArgs {
+    const mgpuSegDesc* segs;
+    const uint32_t* big_list;    // segments for the workgroup routines, largest first
+    const uint32_t* small_list;  // segments for the one-wave routine
+    uint32_t* counters;          // [0] big cursor, [1] small cursor, [2] finished
+                                 // workgroups: zero on entry, left zero
+    uint32_t* error_flag;
+    uint32_t num_big;
+    uint32_t num_small;
+    uint32_t num_values;   // value arrays carried (index generation counts 1)
+    uint32_t index_mode;   // generate source indices into vals[0]
+    uint32_t result_side;  // 0 or 1
+    uint32_t num_passes;   // <= MGPU_SEG_MAX_PASSES
+    uint32_t shift[MGPU_SEG_MAX_PASSES];
+    uint32_t mask[MGPU_SEG_MAX_PASSES];
+};
+
 // rank_scatter: one stable digit pass (rank in the tile, decoupled lookback
 // across tiles, LDS-staged scatter).
 struct mgpuPassArgs {

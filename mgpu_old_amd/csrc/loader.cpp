@@ -50,31 +50,30 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
         st = get_fn(e, e->radixModule, name, &e->fnPass[v]);
     }
     if (st) return st;
+    st = mgpu_seg_load(e);
+    if (st) return st;
 
-    const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_MAX_PASSES + 4;
-    if (hipMalloc(&e->fixedScratch, words * sizeof(uint32_t)) != hipSuccess) {
-        e->fixedScratch = nullptr;
+    st = mgpu_lane_init(&e->lane);
+    if (st) return st;
+    if (hipMalloc(&e->errorFlag, 16) != hipSuccess) {
+        e->errorFlag = nullptr;
         return SORT_STATUS_DEVICE_ALLOC_FAILED;
     }
-    if (hipMemset(e->fixedScratch, 0, words * sizeof(uint32_t)) != hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
-    e->hist = e->fixedScratch;
-    e->digitStart = e->hist + MGPU_MAX_PASSES * MGPU_RADIX;
-    e->tileCounters = e->digitStart + MGPU_MAX_PASSES * MGPU_RADIX;
-    e->errorFlag = e->tileCounters + MGPU_MAX_PASSES;
+    if (hipMemset(e->errorFlag, 0, 16) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
     return SORT_STATUS_SUCCESS;
 }
 
 void mgpu_engine_destroy(sortEngine_d* e) {
     if (!e) return;
-    if (e->stream || e->radixModule) (void)hipStreamSynchronize(e->stream);
+    if (e->lane.stream || e->radixModule) (void)hipStreamSynchronize(e->lane.stream);
     for (auto& t : e->timed) {
         (void)hipEventDestroy(t.start);
         (void)hipEventDestroy(t.stop);
     }
     for (hipEvent_t ev : e->eventPool) (void)hipEventDestroy(ev);
-    if (e->lookback) (void)hipFree(e->lookback);
-    if (e->fixedScratch) (void)hipFree(e->fixedScratch);
+    mgpu_lane_free(&e->lane);
+    if (e->errorFlag) (void)hipFree(e->errorFlag);
     if (e->radixModule) (void)hipModuleUnload(e->radixModule);
+    mgpu_seg_unload(e);
     delete e;
 }

@@ -1,0 +1,492 @@
+// Segmented sort host side: work lists, size-bucket tuning table, one
+// persistent launch per kernel variant, hipGraph capture for replay.
+//
+// No reference counterpart (the reference sorts one array per call,
+// sort/src/mgpusort/mgpusort.cpp:607-616); each segment's result is defined
+// as what sortArray would leave for it: stable by the bit field, values
+// following, on side (passes & 1) of its buffers with passes =
+// sortPlanPasses(endBit - firstBit).
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "engine_internal.h"
+#include "segmented_internal.h"
+
+namespace {
+
+// Segments above this size go to the device-wide path (one rank_scatter
+// launch per pass) instead of one workgroup.
+constexpr uint32_t kDeviceWideMin = 1u << 18;
+
+const SegTuneEntry kDefaultTable[] = {
+    // max log2(n), radix bits, items per thread
+    {6, 8, 8}, {10, 8, 16}, {12, 8, 16}, {31, 8, 16},
+};
+
+bool parse_variant(const char* s, int* rb, int* ipt) {
+    return s && std::sscanf(s, "r%di%d", rb, ipt) == 2;
+}
+
+}  // namespace
+
+sortStatus_t mgpu_seg_load(sortEngine_d* e) {
+    std::string path = e->kernelPath + "mgpusort_seg.hsaco";
+    if (hipModuleLoad(&e->segModule, path.c_str()) != hipSuccess) {
+        e->segModule = nullptr;
+        return SORT_STATUS_KERNEL_NOT_FOUND;
+    }
+    for (int rb = kSegMinBits; rb <= kSegMaxBits; ++rb)
+        for (int k = 0; k < 2; ++k) {
+            int ipt = k ? 16 : 8;
+            char name[64];
+            std::snprintf(name, sizeof(name), "mgpu_segsort_r%d_i%d", rb, ipt);
+            hipFunction_t f = nullptr;
+            if (hipModuleGetFunction(&f, e->segModule, name) != hipSuccess)
+                return SORT_STATUS_KERNEL_ERROR;
+            e->kernelNames[f] = name;
+            int blocks = 0;
+            if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 256, 0) !=
+                    hipSuccess || blocks < 1)
+                blocks = 1;
+            e->segFn[rb - kSegMinBits][k] = f;
+            e->segOccupancy[rb - kSegMinBits][k] = blocks;
+        }
+
+    // Tuning table: kernelPath + "mgpusort_seg_tuning.txt" when present
+    // (written by the autotuner), else the built-in default.
+    e->segTable.assign(std::begin(kDefaultTable), std::end(kDefaultTable));
+    std::ifstream f(e->kernelPath + "mgpusort_seg_tuning.txt");
+    if (f) {
+        std::vector<SegTuneEntry> t;
+        std::string line;
+        while (std::getline(f, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            std::istringstream is(line);
+            SegTuneEntry en;
+            if (is >> en.maxLog2 >> en.radixBits >> en.itemsPerThread) {
+                if (en.radixBits < kSegMinBits || en.radixBits > kSegMaxBits ||
+                    (en.itemsPerThread != 8 && en.itemsPerThread != 16))
+                    continue;
+                t.push_back(en);
+            }
+        }
+        if (!t.empty()) {
+            std::sort(t.begin(), t.end(), [](const SegTuneEntry& a, const SegTuneEntry& b) {
+                return a.maxLog2 < b.maxLog2;
+            });
+            t.back().maxLog2 = 31;
+            e->segTable = t;
+        }
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+void mgpu_seg_unload(sortEngine_d* e) {
+    if (e->segModule) (void)hipModuleUnload(e->segModule);
+    e->segModule = nullptr;
+}
+
+namespace {
+
+int ceil_log2(uint32_t n) {
+    int l = 0;
+    while ((1ull << l) < n) ++l;
+    return l;
+}
+
+void pick_variant(const sortEngine_d* e, uint32_t n, int* rb, int* ipt) {
+    const char* forced = std::getenv("MGPUSORT_SEG_VARIANT");
+    if (parse_variant(forced, rb, ipt) && *rb >= kSegMinBits && *rb <= kSegMaxBits &&
+        (*ipt == 8 || *ipt == 16))
+        return;
+    int l = ceil_log2(n);
+    for (const SegTuneEntry& t : e->segTable)
+        if (l <= t.maxLog2) {
+            *rb = t.radixBits;
+            *ipt = t.itemsPerThread;
+            return;
+        }
+    *rb = 8;
+    *ipt = 16;
+}
+
+}  // namespace
+
+// Builds the launches of one segmented sort.  Device-side arrays are owned by
+// the plan; nothing is launched here.
+sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
+                            int numSegments, int valueCount, int firstBit, int endBit,
+                            SegPlanData* out) {
+    SegPlanData& P = *out;
+    P.engine = e;
+    P.valueCount = valueCount;
+    P.firstBit = firstBit;
+    P.numBits = endBit - firstBit;
+    P.sortPasses = mgpu_plan_passes(P.numBits, MGPU_RADIX_BITS, P.sortWidths);
+    P.resultSide = P.sortPasses & 1;
+    P.hostSegs.assign(segs, segs + numSegments);
+
+    // Group segments by kernel variant; very large ones go device-wide.
+    struct Group {
+        std::vector<uint32_t> big, small;
+    };
+    Group groups[kSegVariants];
+    for (int i = 0; i < numSegments; ++i) {
+        const uint32_t n = segs[i].n;
+        if (n == 0 || P.numBits == 0) continue;
+        if (n >= kDeviceWideMin) {
+            P.wide.push_back((uint32_t)i);
+            continue;
+        }
+        int rb, ipt;
+        pick_variant(e, n, &rb, &ipt);
+        Group& g = groups[seg_variant_index(rb, ipt)];
+        if (n <= 64u * (uint32_t)ipt) g.small.push_back((uint32_t)i);
+        else g.big.push_back((uint32_t)i);
+    }
+
+    // Device arrays: descriptors, then per group [big list | small list],
+    // then 4 counters per group.
+    std::vector<uint32_t> lists;
+    for (int v = 0; v < kSegVariants; ++v) {
+        Group& g = groups[v];
+        if (g.big.empty() && g.small.empty()) continue;
+        std::stable_sort(g.big.begin(), g.big.end(),
+                         [&](uint32_t a, uint32_t b) { return segs[a].n > segs[b].n; });
+        SegLaunch L;
+        L.variant = v;
+        L.bigOffset = (uint32_t)lists.size();
+        lists.insert(lists.end(), g.big.begin(), g.big.end());
+        L.smallOffset = (uint32_t)lists.size();
+        lists.insert(lists.end(), g.small.begin(), g.small.end());
+        L.numBig = (uint32_t)g.big.size();
+        L.numSmall = (uint32_t)g.small.size();
+        const int rb = seg_variant_bits(v), ipt = seg_variant_ipt(v);
+        L.fn = e->segFn[rb - kSegMinBits][ipt == 16];
+        const uint32_t work = L.numBig + (L.numSmall + 3) / 4;
+        const uint32_t maxGrid = (uint32_t)(e->segOccupancy[rb - kSegMinBits][ipt == 16] * e->numCUs);
+        L.grid = std::max(1u, std::min(work, maxGrid));
+        L.passes = mgpu_plan_passes(P.numBits, rb, L.widths);
+        P.launches.push_back(L);
+    }
+
+    const size_t descBytes = sizeof(mgpuSegDesc) * (size_t)std::max(1, numSegments);
+    const size_t listBytes = sizeof(uint32_t) * std::max<size_t>(1, lists.size());
+    const size_t ctrBytes = sizeof(uint32_t) * 4 * std::max<size_t>(1, P.launches.size());
+    const size_t total = descBytes + listBytes + ctrBytes;
+    if (hipMalloc(&P.devBlock, total) != hipSuccess) {
+        P.devBlock = nullptr;
+        return SORT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    char* base = static_cast<char*>(P.devBlock);
+    P.devSegs = reinterpret_cast<mgpuSegDesc*>(base);
+    P.devLists = reinterpret_cast<uint32_t*>(base + descBytes);
+    P.devCounters = reinterpret_cast<uint32_t*>(base + descBytes + listBytes);
+    if (numSegments &&
+        hipMemcpy(P.devSegs, segs, sizeof(mgpuSegDesc) * numSegments, hipMemcpyHostToDevice) !=
+            hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    if (!lists.empty() && hipMemcpy(P.devLists, lists.data(), sizeof(uint32_t) * lists.size(),
+                                    hipMemcpyHostToDevice) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    if (hipMemset(P.devCounters, 0, ctrBytes) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+
+    // Device-wide segments need lookback words before any capture.
+    uint32_t maxWide = 0;
+    for (uint32_t i : P.wide) maxWide = std::max(maxWide, segs[i].n);
+    if (maxWide) {
+        sortStatus_t st = mgpu_ensure_lookback(e, lane, maxWide, P.sortPasses);
+        if (st) return st;
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+// Queues the launches of a built plan on the engine's stream.
+sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
+    sortEngine_d* e = P.engine;
+    for (size_t li = 0; li < P.launches.size(); ++li) {
+        const SegLaunch& L = P.launches[li];
+        mgpuSegArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.segs = P.devSegs;
+        a.big_list = P.devLists + L.bigOffset;
+        a.small_list = P.devLists + L.smallOffset;
+        a.counters = P.devCounters + 4 * li;
+        a.error_flag = e->errorFlag;
+        a.num_big = L.numBig;
+        a.num_small = L.numSmall;
+        a.num_values = P.valueCount == -1 ? 1u : (uint32_t)P.valueCount;
+        a.index_mode = P.valueCount == -1;
+        a.result_side = (uint32_t)P.resultSide;
+        a.num_passes = (uint32_t)L.passes;
+        int bit = P.firstBit;
+        for (int p = 0; p < L.passes; ++p) {
+            a.shift[p] = (uint32_t)bit;
+            a.mask[p] = (1u << L.widths[p]) - 1u;
+            bit += L.widths[p];
+        }
+        sortStatus_t st = mgpu_launch(e, lane.stream, L.fn, L.grid, 256, &a);
+        if (st) return st;
+    }
+    for (uint32_t i : P.wide) {
+        const mgpuSegDesc& s = P.hostSegs[i];
+        uint32_t* keyBuf[2] = {s.keys[0], s.keys[1]};
+        uint32_t* valBuf[MGPU_MAX_VALUES][2];
+        for (int v = 0; v < MGPU_MAX_VALUES; ++v) {
+            valBuf[v][0] = s.vals[v][0];
+            valBuf[v][1] = s.vals[v][1];
+        }
+        sortStatus_t st = mgpu_sort_passes(e, lane, s.n, P.firstBit, P.sortPasses, P.sortWidths,
+                                           P.valueCount, keyBuf, valBuf);
+        if (st) return st;
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+void mgpu_seg_release(SegPlanData& P) {
+    if (P.graphExec) (void)hipGraphExecDestroy(P.graphExec);
+    if (P.graph) (void)hipGraphDestroy(P.graph);
+    if (P.devBlock) (void)hipFree(P.devBlock);
+    P.graphExec = nullptr;
+    P.graph = nullptr;
+    P.devBlock = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Public entry points.
+
+namespace {
+
+sortStatus_t validate_segments(const sortData_d* d, int numSegments, const int* starts) {
+    if (!d || numSegments < 0 || (numSegments > 0 && !starts)) return SORT_STATUS_INVALID_VALUE;
+    if (d->firstBit < 0 || d->endBit > 32 || d->endBit < d->firstBit) return SORT_STATUS_INVALID_VALUE;
+    if (d->valueCount < -1 || d->valueCount > 6) return SORT_STATUS_INVALID_VALUE;
+    if (d->numElements < 0 || d->numElements > d->maxElements) return SORT_STATUS_INVALID_VALUE;
+    if (numSegments > 0) {
+        if (starts[0] < 0) return SORT_STATUS_INVALID_VALUE;
+        for (int i = 0; i < numSegments; ++i)
+            if (starts[i + 1] < starts[i]) return SORT_STATUS_INVALID_VALUE;
+        if (starts[numSegments] > d->numElements) return SORT_STATUS_INVALID_VALUE;
+    }
+    if (!d->keys[0] || !d->keys[1]) return SORT_STATUS_INVALID_VALUE;
+    const int carried = d->valueCount == -1 ? 1 : d->valueCount;
+    for (int v = 0; v < carried; ++v)
+        if (!d->values[v][1] || (d->valueCount != -1 && !d->values[v][0]))
+            return SORT_STATUS_INVALID_VALUE;
+    return SORT_STATUS_SUCCESS;
+}
+
+std::vector<mgpuSegDesc> descs_from_data(const sortData_d* d, int numSegments, const int* starts) {
+    std::vector<mgpuSegDesc> segs((size_t)numSegments);
+    const int carried = d->valueCount == -1 ? 1 : d->valueCount;
+    for (int i = 0; i < numSegments; ++i) {
+        mgpuSegDesc& s = segs[i];
+        std::memset(&s, 0, sizeof(s));
+        const size_t off = (size_t)starts[i];
+        for (int side = 0; side < 2; ++side) {
+            s.keys[side] = reinterpret_cast<uint32_t*>(d->keys[side]) + off;
+            for (int v = 0; v < carried; ++v)
+                if (d->values[v][side])
+                    s.vals[v][side] = reinterpret_cast<uint32_t*>(d->values[v][side]) + off;
+        }
+        s.n = (uint32_t)(starts[i + 1] - starts[i]);
+    }
+    return segs;
+}
+
+void finish_swap(sortData_d* d, int resultSide) {
+    if (resultSide) {
+        std::swap(d->keys[0], d->keys[1]);
+        for (int v = 0; v < MGPU_MAX_VALUES; ++v) std::swap(d->values[v][0], d->values[v][1]);
+        d->parity ^= 1;
+    }
+}
+
+}  // namespace
+
+// A reusable segmented sort.  The graph holds buffer addresses, so one graph
+// is captured per orientation of the ping-pong (orientation 1 = sides
+// swapped, which is where the data is after an odd-pass execute).
+struct sortSegPlan_d {
+    sortEngine_d* engine = nullptr;
+    sortData_d layout;          // buffers as at creation (orientation 0)
+    std::vector<int> starts;
+    int numSegments = 0;
+    SegPlanData data[2];
+    bool built[2] = {false, false};
+    hipStream_t captureStream = nullptr;
+};
+
+namespace {
+
+sortStatus_t build_orientation(sortSegPlan_d* p, int o) {
+    sortData_d d = p->layout;
+    if (o) {
+        std::swap(d.keys[0], d.keys[1]);
+        for (int v = 0; v < MGPU_MAX_VALUES; ++v) std::swap(d.values[v][0], d.values[v][1]);
+    }
+    std::vector<mgpuSegDesc> segs = descs_from_data(&d, p->numSegments, p->starts.data());
+    sortEngine_d* e = p->engine;
+    SegPlanData& P = p->data[o];
+    sortStatus_t st = mgpu_seg_build(e, e->lane, segs.data(), p->numSegments, d.valueCount,
+                                     d.firstBit, d.endBit, &P);
+    if (st) return st;
+    SortLane& lane = e->lane;
+    if (lane.histDirty) {  // no memset nodes inside the graph
+        if (hipMemset(lane.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t)) !=
+                hipSuccess ||
+            hipMemset(lane.tileCounters, 0, MGPU_MAX_PASSES * sizeof(uint32_t)) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        lane.histDirty = false;
+    }
+    if (!p->captureStream &&
+        hipStreamCreateWithFlags(&p->captureStream, hipStreamNonBlocking) != hipSuccess) {
+        p->captureStream = nullptr;
+        return SORT_STATUS_DEVICE_ERROR;
+    }
+    hipStream_t cs = p->captureStream;
+    hipStream_t saved = lane.stream;
+    const bool timing = e->timing;
+    lane.stream = cs;
+    e->timing = false;
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        st = SORT_STATUS_DEVICE_ERROR;
+    } else {
+        sortStatus_t est = mgpu_seg_enqueue(P, lane);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(cs, &g);
+        P.graph = g;
+        if (est) st = est;
+        else if (ce != hipSuccess || !g) st = SORT_STATUS_DEVICE_ERROR;
+        else if (hipGraphInstantiate(&P.graphExec, g, nullptr, nullptr, 0) != hipSuccess)
+            st = SORT_STATUS_DEVICE_ERROR;
+    }
+    lane.stream = saved;
+    e->timing = timing;
+    if (!st) p->built[o] = true;
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+sortStatus_t SORTAPI sortSegmentedArray(sortEngine_t engine, sortData_t data, int numSegments,
+                                        const int* segmentStarts) {
+    if (!engine) return SORT_STATUS_INVALID_VALUE;
+    sortStatus_t st = validate_segments(data, numSegments, segmentStarts);
+    if (st) return st;
+    if (data->endBit == data->firstBit || numSegments == 0) return SORT_STATUS_SUCCESS;
+    std::vector<mgpuSegDesc> segs = descs_from_data(data, numSegments, segmentStarts);
+    SegPlanData P;
+    st = mgpu_seg_build(engine, engine->lane, segs.data(), numSegments, data->valueCount,
+                        data->firstBit, data->endBit, &P);
+    if (!st) st = mgpu_seg_enqueue(P, engine->lane);
+    if (!st && hipStreamSynchronize(engine->lane.stream) != hipSuccess)
+        st = SORT_STATUS_DEVICE_ERROR;
+    mgpu_seg_release(P);
+    if (!st) finish_swap(data, P.resultSide);
+    return st;
+}
+
+sortStatus_t SORTAPI sortSegmentedBatch(sortEngine_t engine, const sortSegment_d* segments,
+                                        int numSegments, int valueCount, int firstBit, int endBit) {
+    if (!engine || numSegments < 0 || (numSegments && !segments)) return SORT_STATUS_INVALID_VALUE;
+    if (firstBit < 0 || endBit > 32 || endBit < firstBit || valueCount < -1 || valueCount > 6)
+        return SORT_STATUS_INVALID_VALUE;
+    if (endBit == firstBit || numSegments == 0) return SORT_STATUS_SUCCESS;
+    std::vector<mgpuSegDesc> segs((size_t)numSegments);
+    const int carried = valueCount == -1 ? 1 : valueCount;
+    for (int i = 0; i < numSegments; ++i) {
+        const sortSegment_d& in = segments[i];
+        mgpuSegDesc& s = segs[i];
+        std::memset(&s, 0, sizeof(s));
+        if (in.numElements < 0 || in.numElements > SORT_MAX_ELEMENTS)
+            return SORT_STATUS_INVALID_VALUE;
+        if (in.numElements && (!in.keys[0] || !in.keys[1])) return SORT_STATUS_INVALID_VALUE;
+        for (int side = 0; side < 2; ++side) {
+            s.keys[side] = reinterpret_cast<uint32_t*>(in.keys[side]);
+            for (int v = 0; v < carried; ++v) {
+                if (in.numElements && !in.values[v][side] && !(valueCount == -1 && side == 0))
+                    return SORT_STATUS_INVALID_VALUE;
+                s.vals[v][side] = reinterpret_cast<uint32_t*>(in.values[v][side]);
+            }
+        }
+        s.n = (uint32_t)in.numElements;
+    }
+    SegPlanData P;
+    sortStatus_t st = mgpu_seg_build(engine, engine->lane, segs.data(), numSegments, valueCount,
+                                     firstBit, endBit, &P);
+    if (!st) st = mgpu_seg_enqueue(P, engine->lane);
+    if (!st && hipStreamSynchronize(engine->lane.stream) != hipSuccess)
+        st = SORT_STATUS_DEVICE_ERROR;
+    mgpu_seg_release(P);
+    return st;
+}
+
+sortStatus_t SORTAPI sortSegmentedPlanCreate(sortEngine_t engine, const sortData_d* data,
+                                             int numSegments, const int* segmentStarts,
+                                             sortSegPlan_t* plan) {
+    if (!engine || !plan) return SORT_STATUS_INVALID_VALUE;
+    *plan = nullptr;
+    sortStatus_t st = validate_segments(data, numSegments, segmentStarts);
+    if (st) return st;
+    sortSegPlan_d* p = new (std::nothrow) sortSegPlan_d();
+    if (!p) return SORT_STATUS_HOST_ALLOC_FAILED;
+    p->engine = engine;
+    p->layout = *data;
+    p->numSegments = numSegments;
+    if (numSegments) p->starts.assign(segmentStarts, segmentStarts + numSegments + 1);
+    if (data->endBit > data->firstBit && numSegments > 0) st = build_orientation(p, 0);
+    if (st) {
+        sortSegmentedPlanDestroy(p);
+        return st;
+    }
+    *plan = p;
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortSegmentedPlanExecute(sortSegPlan_t plan, sortData_t data) {
+    if (!plan || !data) return SORT_STATUS_INVALID_VALUE;
+    const sortData_d& L = plan->layout;
+    if (L.endBit == L.firstBit || plan->numSegments == 0) return SORT_STATUS_SUCCESS;
+    // The data must hold the plan's buffers, in either orientation.
+    int o;
+    if (data->keys[0] == L.keys[0] && data->keys[1] == L.keys[1]) o = 0;
+    else if (data->keys[0] == L.keys[1] && data->keys[1] == L.keys[0]) o = 1;
+    else return SORT_STATUS_INVALID_VALUE;
+    if (!plan->built[o]) {
+        sortStatus_t st = build_orientation(plan, o);
+        if (st) return st;
+    }
+    SegPlanData& P = plan->data[o];
+    sortEngine_d* e = plan->engine;
+    if (e->timing) {
+        // Per-kernel timing cannot look inside a graph: issue the launches
+        // directly so that each one is event-timed.
+        sortStatus_t st = mgpu_seg_enqueue(P, e->lane);
+        if (st) return st;
+    } else if (hipGraphLaunch(P.graphExec, e->lane.stream) != hipSuccess) {
+        return SORT_STATUS_LAUNCH_ERROR;
+    }
+    finish_swap(data, P.resultSide);
+    return SORT_STATUS_SUCCESS;
+}
+
+sortStatus_t SORTAPI sortSegmentedPlanDestroy(sortSegPlan_t plan) {
+    if (!plan) return SORT_STATUS_SUCCESS;
+    if (plan->engine) (void)hipStreamSynchronize(plan->engine->lane.stream);
+    if (plan->captureStream) (void)hipStreamSynchronize(plan->captureStream);
+    for (int o = 0; o < 2; ++o) mgpu_seg_release(plan->data[o]);
+    if (plan->captureStream) (void)hipStreamDestroy(plan->captureStream);
+    delete plan;
+    return SORT_STATUS_SUCCESS;
+}
+
+}  // extern "C"

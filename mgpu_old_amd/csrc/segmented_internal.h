@@ -1,0 +1,69 @@
+// Segmented-sort state shared between the engine and segmented.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/mgpusort.h"
+#include "kernel_args.h"
+
+struct sortEngine_d;
+struct SortLane;
+
+// Kernel variants of mgpusort_seg.hsaco: radix bits 4..8 x items per thread
+// {8, 16}, named mgpu_segsort_r<bits>_i<items>.
+constexpr int kSegMinBits = 4;
+constexpr int kSegMaxBits = 8;
+constexpr int kSegVariants = (kSegMaxBits - kSegMinBits + 1) * 2;
+
+inline int seg_variant_index(int rb, int ipt) { return (rb - kSegMinBits) * 2 + (ipt == 16); }
+inline int seg_variant_bits(int v) { return kSegMinBits + v / 2; }
+inline int seg_variant_ipt(int v) { return (v & 1) ? 16 : 8; }
+
+// One row of the tuning table: segments with ceil(log2 n) <= maxLog2 use
+// this variant (rows sorted by maxLog2).
+struct SegTuneEntry {
+    int maxLog2;
+    int radixBits;
+    int itemsPerThread;
+};
+
+// One persistent launch of one kernel variant.
+struct SegLaunch {
+    int variant = 0;
+    hipFunction_t fn = nullptr;
+    uint32_t grid = 0;
+    uint32_t bigOffset = 0, smallOffset = 0;
+    uint32_t numBig = 0, numSmall = 0;
+    int passes = 0;
+    int widths[MGPU_SEG_MAX_PASSES] = {};
+};
+
+// Everything one segmented sort launches, plus its captured graph.
+struct SegPlanData {
+    sortEngine_d* engine = nullptr;
+    int valueCount = 0;
+    int firstBit = 0;
+    int numBits = 0;
+    int sortPasses = 0;  // passes sortArray would run: fixes the result side
+    int sortWidths[MGPU_MAX_PASSES] = {};
+    int resultSide = 0;
+    std::vector<mgpuSegDesc> hostSegs;
+    std::vector<SegLaunch> launches;
+    std::vector<uint32_t> wide;  // segments sorted by the device-wide path
+    void* devBlock = nullptr;
+    mgpuSegDesc* devSegs = nullptr;
+    uint32_t* devLists = nullptr;
+    uint32_t* devCounters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graphExec = nullptr;
+};
+
+sortStatus_t mgpu_seg_load(sortEngine_d* e);
+void mgpu_seg_unload(sortEngine_d* e);
+sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
+                            int numSegments, int valueCount, int firstBit, int endBit,
+                            SegPlanData* out);
+sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane);
+void mgpu_seg_release(SegPlanData& P);

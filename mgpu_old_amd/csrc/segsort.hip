@@ -1,0 +1,417 @@
+// Segmented LSD radix sort for gfx950: thousands of independent arrays of
+// uneven sizes sorted in one persistent launch.  Each segment's result is
+// exactly what sortArray would produce on it alone (stable by the bit field,
+// values following), left on side `result_side` of its ping-pong buffers.
+//
+// No reference counterpart: the reference sorts one array per sortArray call
+// (sort/src/mgpusort/mgpusort.cpp:607-616); its per-block ranking and scatter
+// steps (sort/src/kernels/sort.cu:54-479) appear here inside one workgroup.
+//
+// Work is split by size on the host:
+//   small  (n <= 64*IPT)     one wave sorts the segment in LDS, all passes;
+//   block  (n <= 256*IPT)    one workgroup sorts it in LDS, all passes;
+//   global (larger)          one workgroup runs every pass through global
+//                            memory, tile by tile, with running digit offsets
+//                            (no cross-workgroup lookback needed).
+// Workgroups first drain the block/global list (largest first), then their
+// waves drain the small list one segment per wave.  The last workgroup to
+// finish puts the queue cursors back to zero, so a captured graph of this
+// launch can be replayed as is.
+//
+// In-LDS routines carry (key, source index) through the passes and gather
+// every value array once at the end; the global routine carries the values
+// through each pass.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernel_args.h"
+#include "radix_device.h"
+
+namespace {
+
+using mgpu::block_exclusive_scan;
+using mgpu::st_agent;
+using mgpu::wave_inclusive_scan;
+using mgpu::wave_rank;
+
+constexpr int BLOCK = 256;
+constexpr int W = BLOCK / 64;
+
+// Orders this wave's LDS accesses before and after the call.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int RB, int IPT>
+struct SegShared {
+    static constexpr int RADIX = 1 << RB;
+    static constexpr int TILE = BLOCK * IPT;
+    static constexpr int MAXP = (32 + RB - 1) / RB;
+    uint32_t stage[TILE];
+    uint32_t count[W][RADIX];
+    uint64_t match[W][RADIX];
+    uint32_t base[MAXP][RADIX];  // global routine: per-pass digit starts
+    uint32_t run[RADIX];         // global routine: running digit offsets
+    uint32_t dst[RADIX];         // global routine: output base per digit for a tile
+    uint32_t tmp[W];
+    uint32_t item;
+};
+
+// Writes the sorted keys and gathers every value array through the source
+// indices: out[pos] = in[src[pos]].  `sync` orders the gathers of the whole
+// group before any store (the result side may be the input side).
+template <int IPT, typename Sync>
+__device__ __forceinline__ void emit(const mgpuSegArgs& a, const mgpuSegDesc& s, uint32_t n,
+                                     const uint32_t* key, const uint32_t* src, const uint32_t* idx,
+                                     Sync sync) {
+    const uint32_t side = a.result_side;
+    uint32_t* ko = s.keys[side];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+        if (idx[j] < n) ko[idx[j]] = key[j];
+    if (a.index_mode) {
+        uint32_t* vo = s.vals[0][side];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) vo[idx[j]] = src[j];
+        return;
+    }
+    for (uint32_t v = 0; v < a.num_values; ++v) {
+        const uint32_t* vi = s.vals[v][0];
+        uint32_t* vo = s.vals[v][side];
+        uint32_t val[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) val[j] = vi[src[j]];
+        sync();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) vo[idx[j]] = val[j];
+        sync();
+    }
+}
+
+// ---- one wave, whole segment in registers + this wave's LDS slice ---------
+template <int RB, int IPT>
+__device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT>& sh,
+                          uint32_t lane, uint32_t wave) {
+    constexpr int RADIX = SegShared<RB, IPT>::RADIX;
+    constexpr int DPL = RADIX >= 64 ? RADIX / 64 : 1;  // digits per lane in the scan
+    const uint32_t n = s.n;
+    uint32_t* stage = sh.stage + wave * 64 * IPT;
+    uint32_t* count = sh.count[wave];
+    uint64_t* match = sh.match[wave];
+
+    uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
+    const uint32_t* ki = s.keys[0];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        idx[j] = j * 64 + lane;
+        src[j] = idx[j];
+        key[j] = idx[j] < n ? ki[idx[j]] : 0u;
+    }
+    for (uint32_t p = 0; p < a.num_passes; ++p) {
+        const uint32_t shift = a.shift[p], mask = a.mask[p];
+#pragma unroll
+        for (int i = 0; i < DPL; ++i)
+            if (lane * DPL + i < (uint32_t)RADIX) count[lane * DPL + i] = 0;
+        wave_sync();
+        uint32_t rank[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) rank[j] = wave_rank((key[j] >> shift) & mask, lane, count, match);
+        wave_sync();
+        // Exclusive scan of the digit counts across the wave.
+        uint32_t c[DPL], sum = 0;
+#pragma unroll
+        for (int i = 0; i < DPL; ++i) {
+            c[i] = lane * DPL + i < (uint32_t)RADIX ? count[lane * DPL + i] : 0u;
+            sum += c[i];
+        }
+        uint32_t run = wave_inclusive_scan(sum, lane) - sum;
+#pragma unroll
+        for (int i = 0; i < DPL; ++i)
+            if (lane * DPL + i < (uint32_t)RADIX) {
+                count[lane * DPL + i] = run;
+                run += c[i];
+            }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) {
+                pos[j] = count[(key[j] >> shift) & mask] + rank[j];
+                stage[pos[j]] = key[j];
+            }
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) key[j] = stage[idx[j]];
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) stage[pos[j]] = src[j];
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) src[j] = stage[idx[j]];
+        wave_sync();
+    }
+    emit<IPT>(a, s, n, key, src, idx, [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        wave_sync();
+    });
+}
+
+// ---- one workgroup, whole segment in registers + LDS ----------------------
+template <int RB, int IPT>
+__device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT>& sh,
+                           uint32_t tid, uint32_t lane, uint32_t wave) {
+    constexpr int RADIX = SegShared<RB, IPT>::RADIX;
+    const uint32_t n = s.n;
+    uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
+    const uint32_t* ki = s.keys[0];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        idx[j] = wave * 64 * IPT + j * 64 + lane;
+        src[j] = idx[j];
+        key[j] = idx[j] < n ? ki[idx[j]] : 0u;
+    }
+    for (uint32_t p = 0; p < a.num_passes; ++p) {
+        const uint32_t shift = a.shift[p], mask = a.mask[p];
+        for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&sh.count[0][0])[i] = 0;
+        __syncthreads();
+        uint32_t rank[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n)
+                rank[j] = wave_rank((key[j] >> shift) & mask, lane, sh.count[wave], sh.match[wave]);
+        __syncthreads();
+        uint32_t total = 0;
+        if (tid < (uint32_t)RADIX) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                uint32_t c = sh.count[w][tid];
+                sh.count[w][tid] = total;
+                total += c;
+            }
+        }
+        const uint32_t start = block_exclusive_scan<BLOCK>(total, sh.tmp, lane, wave);
+        if (tid < (uint32_t)RADIX) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) sh.count[w][tid] += start;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) {
+                pos[j] = sh.count[wave][(key[j] >> shift) & mask] + rank[j];
+                sh.stage[pos[j]] = key[j];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) key[j] = sh.stage[idx[j]];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) sh.stage[pos[j]] = src[j];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (idx[j] < n) src[j] = sh.stage[idx[j]];
+        __syncthreads();
+    }
+    emit<IPT>(a, s, n, key, src, idx, [] { __syncthreads(); });
+}
+
+// ---- one workgroup, passes through global memory ---------------------------
+template <int RB, int IPT>
+__device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT>& sh,
+                            uint32_t tid, uint32_t lane, uint32_t wave) {
+    constexpr int RADIX = SegShared<RB, IPT>::RADIX;
+    constexpr uint32_t TILE = SegShared<RB, IPT>::TILE;
+    const uint32_t n = s.n;
+    const uint32_t np = a.num_passes;
+
+    // Digit histograms of every pass in one read, then their exclusive scans.
+    for (uint32_t i = tid; i < np * RADIX; i += BLOCK) (&sh.base[0][0])[i] = 0;
+    __syncthreads();
+    for (uint32_t i = tid; i < n; i += BLOCK) {
+        const uint32_t k = s.keys[0][i];
+        for (uint32_t p = 0; p < np; ++p)
+            __hip_atomic_fetch_add(&sh.base[p][(k >> a.shift[p]) & a.mask[p]], 1u,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    for (uint32_t p = 0; p < np; ++p) {
+        uint32_t c = tid < (uint32_t)RADIX ? sh.base[p][tid] : 0u;
+        uint32_t e = block_exclusive_scan<BLOCK>(c, sh.tmp, lane, wave);
+        if (tid < (uint32_t)RADIX) sh.base[p][tid] = e;
+    }
+
+    const uint32_t nv = a.num_values;
+    for (uint32_t p = 0; p < np; ++p) {
+        const uint32_t in = p & 1, out = in ^ 1;
+        const uint32_t shift = a.shift[p], mask = a.mask[p];
+        const uint32_t* ki = s.keys[in];
+        uint32_t* ko = s.keys[out];
+        if (tid < (uint32_t)RADIX) sh.run[tid] = sh.base[p][tid];
+        for (uint32_t tb = 0; tb < n; tb += TILE) {
+            for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&sh.count[0][0])[i] = 0;
+            __syncthreads();
+            uint32_t key[IPT], rank[IPT], pos[IPT], dest[IPT];
+            const uint32_t wb = tb + wave * 64 * IPT;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t i = wb + j * 64 + lane;
+                if (i < n) {
+                    key[j] = ki[i];
+                    rank[j] = wave_rank((key[j] >> shift) & mask, lane, sh.count[wave],
+                                        sh.match[wave]);
+                }
+            }
+            __syncthreads();
+            uint32_t total = 0;
+            if (tid < (uint32_t)RADIX) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) {
+                    uint32_t c = sh.count[w][tid];
+                    sh.count[w][tid] = total;
+                    total += c;
+                }
+            }
+            const uint32_t start = block_exclusive_scan<BLOCK>(total, sh.tmp, lane, wave);
+            if (tid < (uint32_t)RADIX) {
+#pragma unroll
+                for (int w = 0; w < W; ++w) sh.count[w][tid] += start;
+                sh.dst[tid] = sh.run[tid] - start;
+                sh.run[tid] += total;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t i = wb + j * 64 + lane;
+                if (i < n) {
+                    pos[j] = sh.count[wave][(key[j] >> shift) & mask] + rank[j];
+                    sh.stage[pos[j]] = key[j];
+                }
+            }
+            __syncthreads();
+            const uint32_t valid = min(TILE, n - tb);
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t q = j * BLOCK + tid;
+                dest[j] = n;
+                if (q < valid) {
+                    const uint32_t k = sh.stage[q];
+                    dest[j] = sh.dst[(k >> shift) & mask] + q;
+                    if (dest[j] < n) ko[dest[j]] = k;
+                    else st_agent(a.error_flag, 4u);
+                }
+            }
+            for (uint32_t v = 0; v < nv; ++v) {
+                const bool gen = a.index_mode && p == 0;
+                const uint32_t* vi = s.vals[v][in];
+                uint32_t val[IPT];
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const uint32_t i = wb + j * 64 + lane;
+                    if (i < n) val[j] = gen ? i : vi[i];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const uint32_t i = wb + j * 64 + lane;
+                    if (i < n) sh.stage[pos[j]] = val[j];
+                }
+                __syncthreads();
+                uint32_t* vo = s.vals[v][out];
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) {
+                    const uint32_t q = j * BLOCK + tid;
+                    if (q < valid && dest[j] < n) vo[dest[j]] = sh.stage[q];
+                }
+            }
+            __syncthreads();
+        }
+        // This pass's stores must be seen by the next pass's loads of the
+        // same workgroup (the L1 may hold lines read two passes ago).
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+
+    // The result is on side (np & 1); move it when the launch wants the other.
+    if ((np & 1) != a.result_side) {
+        const uint32_t from = np & 1, to = a.result_side;
+        for (uint32_t i = tid; i < n; i += BLOCK) s.keys[to][i] = s.keys[from][i];
+        for (uint32_t v = 0; v < nv; ++v)
+            for (uint32_t i = tid; i < n; i += BLOCK) s.vals[v][to][i] = s.vals[v][from][i];
+    }
+}
+
+template <int RB, int IPT>
+__device__ void segsort_body(const mgpuSegArgs& a) {
+    __shared__ SegShared<RB, IPT> sh;
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (uint32_t i = tid; i < (uint32_t)(W * SegShared<RB, IPT>::RADIX); i += BLOCK)
+        (&sh.match[0][0])[i] = 0;
+
+    // Workgroup phase: block and global segments, largest first.
+    while (true) {
+        if (tid == 0)
+            sh.item = __hip_atomic_fetch_add(&a.counters[0], 1u, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        const uint32_t item = sh.item;
+        __syncthreads();
+        if (item >= a.num_big) break;
+        const mgpuSegDesc& s = a.segs[a.big_list[item]];
+        if (s.n <= (uint32_t)SegShared<RB, IPT>::TILE) block_sort<RB, IPT>(a, s, sh, tid, lane, wave);
+        else global_sort<RB, IPT>(a, s, sh, tid, lane, wave);
+        __syncthreads();
+    }
+
+    // Wave phase: one small segment per wave at a time.
+    while (true) {
+        uint32_t item = 0;
+        if (lane == 0)
+            item = __hip_atomic_fetch_add(&a.counters[1], 1u, __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT);
+        item = __shfl(item, 0, 64);
+        if (item >= a.num_small) break;
+        wave_sort<RB, IPT>(a, a.segs[a.small_list[item]], sh, lane, wave);
+    }
+
+    __syncthreads();
+    if (tid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t done = __hip_atomic_fetch_add(&a.counters[2], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+        if (done == gridDim.x - 1) {  // every workgroup is past both queues
+            st_agent(&a.counters[0], 0u);
+            st_agent(&a.counters[1], 0u);
+            st_agent(&a.counters[2], 0u);
+        }
+    }
+}
+
+}  // namespace
+
+#define MGPU_SEGSORT(RB, IPT)                                                                  \
+    extern "C" __global__ void __launch_bounds__(256) mgpu_segsort_r##RB##_i##IPT(mgpuSegArgs a) { \
+        segsort_body<RB, IPT>(a);                                                              \
+    }
+
+MGPU_SEGSORT(4, 8)
+MGPU_SEGSORT(4, 16)
+MGPU_SEGSORT(5, 8)
+MGPU_SEGSORT(5, 16)
+MGPU_SEGSORT(6, 8)
+MGPU_SEGSORT(6, 16)
+MGPU_SEGSORT(7, 8)
+MGPU_SEGSORT(7, 16)
+MGPU_SEGSORT(8, 8)
+MGPU_SEGSORT(8, 16)
