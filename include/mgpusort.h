@@ -236,6 +236,53 @@ sortStatus_t SORTAPI sortSegmentedPlanCreate(sortEngine_t engine, const sortData
 sortStatus_t SORTAPI sortSegmentedPlanExecute(sortSegPlan_t plan, sortData_t data);
 sortStatus_t SORTAPI sortSegmentedPlanDestroy(sortSegPlan_t plan);
 
+/* ---- Sort service (no counterpart in the reference). -----------------------
+ * Continuous batching of sort requests: requests queue in arrival order, are
+ * admitted while the ping-pong scratch they need (4 * numElements * (1 +
+ * value arrays) bytes) fits in a fixed HBM budget, and every admitted request
+ * is packed into the segmented kernels of the next free in-flight slot, each
+ * slot with its own stream, so batches overlap on the device.  Each request is
+ * sorted in place, exactly as sortArray would sort it alone. */
+typedef struct {
+    CUdeviceptr keys;       /* numElements keys, sorted in place */
+    CUdeviceptr values[6];  /* the first (valueCount == -1 ? 1 : valueCount) are used */
+    int numElements;
+    int valueCount;         /* -1 (indices into values[0]) .. 6 */
+    int firstBit;
+    int endBit;
+    unsigned long long tag; /* returned with the completion */
+} sortRequest_d;
+
+typedef struct {
+    unsigned long long tag;
+    sortStatus_t status;
+    int batch;       /* sequence number of the batch that ran it (-1: none) */
+    double submitMs; /* host clock, milliseconds since the server started */
+    double launchMs;
+    double doneMs;
+} sortCompletion_d;
+
+struct sortServer_d;
+typedef struct sortServer_d* sortServer_t;
+
+/* hbmBudgetBytes: scratch slab for admitted requests.  maxInFlight: batches
+ * on the device at once (1..16).  maxElements: largest request accepted. */
+sortStatus_t SORTAPI sortServerCreate(sortEngine_t engine, unsigned long long hbmBudgetBytes,
+                                      int maxInFlight, int maxElements, sortServer_t* server);
+/* Thread-safe.  The buffers must stay untouched until the completion. */
+sortStatus_t SORTAPI sortServerSubmit(sortServer_t server, const sortRequest_d* request);
+/* Up to maxCount completions, waiting at most timeoutMs for the first;
+ * returns how many were written (-1 on a bad argument). */
+int SORTAPI sortServerPoll(sortServer_t server, sortCompletion_d* out, int maxCount,
+                           int timeoutMs);
+/* Load driver: submits requests[i] (tag = i) at arrivalMs[i] after the call
+ * starts, waits for all of them and writes completions[i]. */
+sortStatus_t SORTAPI sortServerReplay(sortServer_t server, const sortRequest_d* requests,
+                                      const double* arrivalMs, int count,
+                                      sortCompletion_d* completions);
+/* Finishes everything submitted, then frees the server. */
+sortStatus_t SORTAPI sortServerDestroy(sortServer_t server);
+
 /* Waits for the engine's stream and reports SORT_STATUS_KERNEL_ERROR if a
  * kernel detected an inconsistency (a scatter offset out of range, or a
  * lookback wait that gave up) since the last call; SUCCESS otherwise. */

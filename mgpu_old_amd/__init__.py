@@ -11,7 +11,8 @@ import os
 
 __all__ = [
     "SortStatus", "MgpuError", "SortData", "lib", "library_path", "kernel_path",
-    "Engine", "MgpuSortData", "plan_passes", "SortSegment", "SegmentedPlan",
+    "Engine", "MgpuSortData", "plan_passes", "SortSegment", "SegmentedPlan", "SortRequest",
+    "SortCompletion", "Server",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -74,6 +75,31 @@ class SortSegment(ctypes.Structure):
     ]
 
 
+class SortRequest(ctypes.Structure):
+    """sortRequest_d: one request to the sort server (sorted in place)."""
+    _fields_ = [
+        ("keys", ctypes.c_uint64),
+        ("values", ctypes.c_uint64 * 6),
+        ("numElements", ctypes.c_int),
+        ("valueCount", ctypes.c_int),
+        ("firstBit", ctypes.c_int),
+        ("endBit", ctypes.c_int),
+        ("tag", ctypes.c_ulonglong),
+    ]
+
+
+class SortCompletion(ctypes.Structure):
+    """sortCompletion_d: outcome and host timestamps of one request."""
+    _fields_ = [
+        ("tag", ctypes.c_ulonglong),
+        ("status", ctypes.c_int),
+        ("batch", ctypes.c_int),
+        ("submitMs", ctypes.c_double),
+        ("launchMs", ctypes.c_double),
+        ("doneMs", ctypes.c_double),
+    ]
+
+
 class KernelTiming(ctypes.Structure):
     """sortKernelTiming_t: per-kernel aggregate of event-timed launches."""
     _fields_ = [
@@ -128,6 +154,15 @@ def lib():
                                                        P(ctypes.c_void_p)]),
             "sortSegmentedPlanExecute": (ctypes.c_int, [ctypes.c_void_p, sd]),
             "sortSegmentedPlanDestroy": (ctypes.c_int, [ctypes.c_void_p]),
+            "sortServerCreate": (ctypes.c_int, [eng, ctypes.c_ulonglong, ctypes.c_int,
+                                                ctypes.c_int, P(ctypes.c_void_p)]),
+            "sortServerSubmit": (ctypes.c_int, [ctypes.c_void_p, P(SortRequest)]),
+            "sortServerPoll": (ctypes.c_int, [ctypes.c_void_p, P(SortCompletion), ctypes.c_int,
+                                              ctypes.c_int]),
+            "sortServerReplay": (ctypes.c_int, [ctypes.c_void_p, P(SortRequest),
+                                                P(ctypes.c_double), ctypes.c_int,
+                                                P(SortCompletion)]),
+            "sortServerDestroy": (ctypes.c_int, [ctypes.c_void_p]),
             "sortCheckDevice": (ctypes.c_int, [eng]),
             "sortEnableTiming": (ctypes.c_int, [eng, ctypes.c_bool]),
             "sortReadTiming": (ctypes.c_int, [eng, P(KernelTiming), ctypes.c_int]),
@@ -264,6 +299,53 @@ class SegmentedPlan:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """sortServerCreate / Submit / Poll / Replay / Destroy."""
+
+    def __init__(self, engine, hbm_budget_bytes, max_in_flight=3, max_elements=1 << 24):
+        self._engine = engine
+        self._h = ctypes.c_void_p()
+        _check(lib().sortServerCreate(engine.handle, int(hbm_budget_bytes), int(max_in_flight),
+                                      int(max_elements), ctypes.byref(self._h)),
+               "sortServerCreate")
+
+    def submit(self, request):
+        _check(lib().sortServerSubmit(self._h, ctypes.byref(request)), "sortServerSubmit")
+
+    def poll(self, max_count=256, timeout_ms=0):
+        buf = (SortCompletion * max_count)()
+        n = lib().sortServerPoll(self._h, buf, max_count, int(timeout_ms))
+        if n < 0:
+            raise MgpuError(SortStatus.INVALID_VALUE, "sortServerPoll")
+        return [buf[i] for i in range(n)]
+
+    def replay(self, requests, arrival_ms):
+        """Submit requests[i] at arrival_ms[i]; returns the completions in order."""
+        n = len(requests)
+        reqs = (SortRequest * max(1, n))(*requests)
+        arr = (ctypes.c_double * max(1, n))(*[float(x) for x in arrival_ms])
+        out = (SortCompletion * max(1, n))()
+        _check(lib().sortServerReplay(self._h, reqs, arr, n, out), "sortServerReplay")
+        return [out[i] for i in range(n)]
+
+    def close(self):
+        if self._h:
+            lib().sortServerDestroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class MgpuSortData(SortData):

@@ -24,7 +24,7 @@ CODE_OBJECTS = {
     "mgpusort_radix.hsaco": ["radixsort.hip"],
     "mgpusort_seg.hsaco": ["segsort.hip"],
 }
-HOST_SOURCES = ["engine.cpp", "loader.cpp", "segmented.cpp"]
+HOST_SOURCES = ["engine.cpp", "loader.cpp", "segmented.cpp", "server.cpp"]
 
 
 def _run(cmd):

@@ -122,7 +122,7 @@ void pick_variant(const sortEngine_d* e, uint32_t n, int* rb, int* ipt) {
 // the plan; nothing is launched here.
 sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
                             int numSegments, int valueCount, int firstBit, int endBit,
-                            SegPlanData* out) {
+                            SegPlanData* out, SegArena* arena) {
     SegPlanData& P = *out;
     P.engine = e;
     P.valueCount = valueCount;
@@ -177,25 +177,61 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     }
 
     const size_t descBytes = sizeof(mgpuSegDesc) * (size_t)std::max(1, numSegments);
-    const size_t listBytes = sizeof(uint32_t) * std::max<size_t>(1, lists.size());
-    const size_t ctrBytes = sizeof(uint32_t) * 4 * std::max<size_t>(1, P.launches.size());
-    const size_t total = descBytes + listBytes + ctrBytes;
-    if (hipMalloc(&P.devBlock, total) != hipSuccess) {
-        P.devBlock = nullptr;
-        return SORT_STATUS_DEVICE_ALLOC_FAILED;
+    const size_t listBytes = (sizeof(uint32_t) * std::max<size_t>(1, lists.size()) + 255) & ~255ull;
+    const size_t ctrWords = 4 * std::max<size_t>(1, P.launches.size());
+    const size_t payload = descBytes + listBytes;
+    const size_t total = payload + ctrWords * sizeof(uint32_t);
+    char* base = nullptr;
+    if (!arena) {
+        if (hipMalloc(&P.devBlock, total) != hipSuccess) {
+            P.devBlock = nullptr;
+            return SORT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+        base = static_cast<char*>(P.devBlock);
+        if (numSegments && hipMemcpy(base, segs, sizeof(mgpuSegDesc) * numSegments,
+                                     hipMemcpyHostToDevice) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        if (!lists.empty() && hipMemcpy(base + descBytes, lists.data(),
+                                        sizeof(uint32_t) * lists.size(),
+                                        hipMemcpyHostToDevice) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        if (hipMemset(base + payload, 0, ctrWords * sizeof(uint32_t)) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+    } else {
+        // Counters live at the end of the arena and are only ever left zero
+        // by the kernels, so they are cleared once, when the arena (re)grows.
+        // Layout: [payload capacity = hostBytes][counterWords counters].
+        if (arena->hostBytes < payload || arena->counterWords < ctrWords) {
+            if (hipStreamSynchronize(lane.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+            mgpu_arena_free(arena);
+            const size_t payloadCap = 2 * payload + 4096;
+            const size_t words = 2 * std::max<size_t>(ctrWords, 64);
+            const size_t cap = payloadCap + words * sizeof(uint32_t);
+            if (hipMalloc(&arena->dev, cap) != hipSuccess) {
+                arena->dev = nullptr;
+                return SORT_STATUS_DEVICE_ALLOC_FAILED;
+            }
+            if (hipHostMalloc(&arena->host, payloadCap, hipHostMallocDefault) != hipSuccess) {
+                arena->host = nullptr;
+                mgpu_arena_free(arena);
+                return SORT_STATUS_HOST_ALLOC_FAILED;
+            }
+            arena->devBytes = cap;
+            arena->hostBytes = payloadCap;
+            arena->counterWords = words;
+            if (hipMemset(arena->dev, 0, cap) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+        }
+        base = static_cast<char*>(arena->dev);
+        char* h = static_cast<char*>(arena->host);
+        if (numSegments) std::memcpy(h, segs, sizeof(mgpuSegDesc) * numSegments);
+        if (!lists.empty()) std::memcpy(h + descBytes, lists.data(), sizeof(uint32_t) * lists.size());
+        if (hipMemcpyAsync(base, h, payload, hipMemcpyHostToDevice, lane.stream) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        P.devCounters = reinterpret_cast<uint32_t*>(base + arena->hostBytes);
     }
-    char* base = static_cast<char*>(P.devBlock);
     P.devSegs = reinterpret_cast<mgpuSegDesc*>(base);
     P.devLists = reinterpret_cast<uint32_t*>(base + descBytes);
-    P.devCounters = reinterpret_cast<uint32_t*>(base + descBytes + listBytes);
-    if (numSegments &&
-        hipMemcpy(P.devSegs, segs, sizeof(mgpuSegDesc) * numSegments, hipMemcpyHostToDevice) !=
-            hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
-    if (!lists.empty() && hipMemcpy(P.devLists, lists.data(), sizeof(uint32_t) * lists.size(),
-                                    hipMemcpyHostToDevice) != hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
-    if (hipMemset(P.devCounters, 0, ctrBytes) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    if (!arena) P.devCounters = reinterpret_cast<uint32_t*>(base + payload);
 
     // Device-wide segments need lookback words before any capture.
     uint32_t maxWide = 0;
@@ -247,6 +283,12 @@ sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;
+}
+
+void mgpu_arena_free(SegArena* a) {
+    if (a->dev) (void)hipFree(a->dev);
+    if (a->host) (void)hipHostFree(a->host);
+    *a = SegArena();
 }
 
 void mgpu_seg_release(SegPlanData& P) {

@@ -40,6 +40,17 @@ struct SegLaunch {
     int widths[MGPU_SEG_MAX_PASSES] = {};
 };
 
+// Reusable home for the device arrays of successive builds on one lane
+// (descriptors, work lists, queue counters), filled with async copies from
+// pinned staging so that a build neither allocates nor synchronizes.
+struct SegArena {
+    void* dev = nullptr;
+    size_t devBytes = 0;
+    void* host = nullptr;  // pinned
+    size_t hostBytes = 0;
+    size_t counterWords = 0;  // zeroed counter words at the end of dev
+};
+
 // Everything one segmented sort launches, plus its captured graph.
 struct SegPlanData {
     sortEngine_d* engine = nullptr;
@@ -52,7 +63,7 @@ struct SegPlanData {
     std::vector<mgpuSegDesc> hostSegs;
     std::vector<SegLaunch> launches;
     std::vector<uint32_t> wide;  // segments sorted by the device-wide path
-    void* devBlock = nullptr;
+    void* devBlock = nullptr;  // owned allocation (null when an arena holds the arrays)
     mgpuSegDesc* devSegs = nullptr;
     uint32_t* devLists = nullptr;
     uint32_t* devCounters = nullptr;
@@ -62,8 +73,13 @@ struct SegPlanData {
 
 sortStatus_t mgpu_seg_load(sortEngine_d* e);
 void mgpu_seg_unload(sortEngine_d* e);
+// Plans one segmented sort.  Without an arena the device arrays get an
+// allocation of their own (synchronous copies); with one they are copied
+// asynchronously on lane.stream into the arena, which must not be in use by
+// earlier work still running on another stream.
 sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
                             int numSegments, int valueCount, int firstBit, int endBit,
-                            SegPlanData* out);
+                            SegPlanData* out, SegArena* arena = nullptr);
+void mgpu_arena_free(SegArena* a);
 sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane);
 void mgpu_seg_release(SegPlanData& P);

@@ -1,0 +1,117 @@
+"""Sort server (continuous batching) parity, needs a gfx950 GPU: every served
+request must equal the oracle's stable sort of that request alone, and the
+same request sorted alone by sortArray, whatever it was batched with."""
+import numpy as np
+import pytest
+
+import mgpu_old_amd as mg
+from mgpu_old_amd import workloads
+from gpu_helpers import GpuSort, make_engine, to_dev, to_host
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def engine():
+    eng = make_engine()
+    yield eng
+    eng.close()
+
+
+def _make_requests(count, seed, lo_log2=10, hi_log2=20):
+    sizes = workloads.log_uniform_sizes(count, lo_log2, hi_log2, seed=seed)
+    choice = workloads.words(seed + 1, 0, 2 * count)
+    widths = np.array([16, 24, 32])[choice[:count] % 3]
+    u = choice[count:] % 100
+    vcs = np.where(u < 50, 0, np.where(u < 85, 1, 2))
+    reqs, host, tensors = [], [], []
+    for i in range(count):
+        n, w, vc = int(sizes[i]), int(widths[i]), int(vcs[i])
+        keys = workloads.uniform_keys(n, seed=seed * 1000 + i, mask_bits=w)
+        vals = [workloads.uniform_keys(n, seed=seed * 1000 + 500 + i * 7 + j) for j in range(vc)]
+        tk = to_dev(keys)
+        tv = [to_dev(v) for v in vals]
+        r = mg.SortRequest()
+        r.keys = tk.data_ptr()
+        for j, t in enumerate(tv):
+            r.values[j] = t.data_ptr()
+        r.numElements, r.valueCount, r.firstBit, r.endBit = n, vc, 0, w
+        reqs.append(r)
+        host.append((keys, vals, w))
+        tensors.append((tk, tv))
+    return reqs, host, tensors
+
+
+def _poisson(count, mean_gap_ms, seed):
+    u = (workloads.words(seed, 0, count).astype(np.float64) + 0.5) / 4294967296.0
+    return np.cumsum(-np.log(u) * mean_gap_ms)
+
+
+def test_served_requests_match_oracle_and_alone(engine, oracle):
+    import torch
+    reqs, host, tensors = _make_requests(48, seed=7)
+    arrivals = _poisson(len(reqs), 0.2, seed=8)
+    torch.cuda.synchronize()
+    with mg.Server(engine, 1 << 30, max_in_flight=3, max_elements=1 << 20) as srv:
+        comps = srv.replay(reqs, arrivals)
+    torch.cuda.synchronize()
+    batches = set()
+    for i, c in enumerate(comps):
+        assert c.status == 0, f"request {i} failed: {mg.status_string(c.status)}"
+        assert c.doneMs >= c.launchMs >= c.submitMs - 1e-6
+        batches.add(c.batch)
+        keys, vals, w = host[i]
+        tk, tv = tensors[i]
+        n = keys.size
+        rk, rv = oracle.stable_sort(keys, vals, first_bit=0, end_bit=w)
+        assert np.array_equal(to_host(tk, n), rk), f"request {i} keys differ"
+        for j, t in enumerate(tv):
+            assert np.array_equal(to_host(t, n), rv[j]), f"request {i} values[{j}] differ"
+        if i % 6 == 0:  # the same request alone through sortArray
+            alone = GpuSort(keys, vals, first_bit=0, end_bit=w).run(engine)
+            assert np.array_equal(alone.keys(), rk)
+    assert len(batches) > 1  # work really was split into several batches
+
+
+def test_budget_admission_and_rejection(engine, oracle):
+    import torch
+    reqs, host, tensors = _make_requests(12, seed=21, lo_log2=14, hi_log2=18)
+    big = max(r.numElements * 4 * (1 + max(r.valueCount, 0)) for r in reqs)
+    # Room for about one large request at a time: admission must serialize.
+    budget = big + (1 << 20)
+    too_big = mg.SortRequest()
+    extra = to_dev(workloads.uniform_keys(1 << 20, 3))
+    too_big.keys = extra.data_ptr()
+    too_big.numElements, too_big.valueCount, too_big.firstBit, too_big.endBit = 1 << 20, 0, 0, 32
+    torch.cuda.synchronize()
+    with mg.Server(engine, budget, max_in_flight=2, max_elements=1 << 20) as srv:
+        comps = srv.replay(reqs + [too_big], [0.0] * (len(reqs) + 1))
+    assert comps[-1].status == mg.SortStatus.CONFIG_NOT_SUPPORTED
+    for i, c in enumerate(comps[:-1]):
+        assert c.status == 0
+        keys, vals, w = host[i]
+        rk, rv = oracle.stable_sort(keys, vals, first_bit=0, end_bit=w)
+        assert np.array_equal(to_host(tensors[i][0], keys.size), rk)
+
+
+def test_submit_and_poll(engine, oracle):
+    import torch
+    reqs, host, tensors = _make_requests(10, seed=31, lo_log2=10, hi_log2=16)
+    torch.cuda.synchronize()
+    with mg.Server(engine, 1 << 28, max_in_flight=2, max_elements=1 << 16) as srv:
+        for i, r in enumerate(reqs):
+            r.tag = 100 + i
+            srv.submit(r)
+        got = {}
+        while len(got) < len(reqs):
+            for c in srv.poll(timeout_ms=100):
+                got[c.tag] = c
+        bad = mg.SortRequest()
+        bad.numElements = (1 << 16) + 1
+        with pytest.raises(mg.MgpuError):
+            srv.submit(bad)
+    for i in range(len(reqs)):
+        assert got[100 + i].status == 0
+        keys, vals, w = host[i]
+        rk, _ = oracle.stable_sort(keys, vals, first_bit=0, end_bit=w)
+        assert np.array_equal(to_host(tensors[i][0], keys.size), rk)

@@ -119,7 +119,7 @@ def test_zero_bits_and_zero_elements_do_nothing(engine):
     keys = workloads.uniform_keys(5000, 11)
     g = GpuSort(keys, value_count=-1, first_bit=9, end_bit=9).run(engine)
     assert np.array_equal(g.keys(), keys) and g.data.parity == 0
-    g = GpuSort(np.zeros(0, dtype=np.uint32), value_count=1, capacity=16)
+    g = GpuSort(np.zeros(0, dtype=np.uint32), [np.zeros(0, dtype=np.uint32)], capacity=16)
     g.run(engine)
     assert g.data.parity == 0
 
