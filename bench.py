@@ -1,22 +1,24 @@
 """Benchmark of the gfx950 LSD radix sort through the mgpusort C ABI.
 
-Headline (the `value` of the JSON line): sortArray of 2^28 uint32 keys uniform
+Headline (`value` of the JSON line): sortArray of 2^28 uint32 keys uniform
 over 32 bits plus one uint32 value array (0..N-1), bits [0, 32), Gkeys/s on
-one GPU.  Every timed step sorts its own pre-made copy of the input that is
-already resident in HBM, so the timed region holds exactly `steps` sortArray
-calls and nothing else.
+one GPU.  Every timed step sorts its own pre-made copy of the input, already
+resident in HBM, so the timed region holds exactly `steps` sortArray calls.
 
-Also reported on the same line:
+Also on the same line:
   roofline      the dominant kernel (rank_scatter) from per-launch HIP events:
-                algorithmic bytes 8*N*(1+v) per launch / average launch time
-  sort_roofline whole-sort algorithmic bytes sum_p 4N(3+2v) (index pass 16N)
-                / sort time, against 8 TB/s
+                8*N*(1+v) algorithmic bytes per launch / average launch time
+  sort_roofline the whole sort: sum over passes of 4N(3+2v) (16N for an
+                index-generating pass) / sort time, against 8 TB/s
   cpu_baseline  the oracle's std::stable_sort (1 core) on a bounded sample
-  extra         the other sort workloads (2^20 keys, 2^25 key+value, 2^28
-                low-entropy with index generation at 32 and 24 bits)
+  extra         2^20 keys only, 2^25 key+value, 2^28 low-entropy keys with
+                index generation at 32 and 24 bits
+  segmented     4096 segments (sizes log-uniform in [1, 2^16], key + value)
+                in one graph-replayed launch vs a loop of sortArray calls
+  served        Poisson stream through the sort server: p50/p99 latency and
+                Gkeys/s at offered loads of 50% and 80% of measured capacity
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -37,8 +39,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--log2n", type=int, default=28)
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-segmented", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log2n", type=int, default=25)
+    ap.add_argument("--served-requests", type=int, default=600)
     return ap.parse_args()
 
 
@@ -57,24 +62,23 @@ def sort_bytes(n, value_count, passes):
 
 
 class Workload:
-    """Device buffers for `copies` independent inputs of one sort workload."""
+    """Device buffers for `copies` independent inputs of one sort workload
+    (keys and values given as int32 device tensors)."""
 
-    def __init__(self, torch, keys_np, values_np, value_count, first_bit, end_bit, copies):
+    def __init__(self, torch, keys, values, value_count, first_bit, end_bit, copies):
         self.torch = torch
-        self.n = keys_np.size
+        self.n = keys.numel()
         self.value_count = value_count
         self.first_bit, self.end_bit = first_bit, end_bit
-        dev = torch.device("cuda")
-        self.src_k = torch.from_numpy(keys_np.view(np.int32)).to(dev)
-        self.src_v = [torch.from_numpy(v.view(np.int32)).to(dev) for v in values_np]
+        self.src_k = keys
+        self.src_v = values
         carried = 1 if value_count == -1 else value_count
-        self.scratch_k = torch.empty_like(self.src_k)
-        self.scratch_v = [torch.empty_like(self.src_k) for _ in range(carried)]
+        self.scratch_k = torch.empty_like(keys)
+        self.scratch_v = [torch.empty_like(keys) for _ in range(carried)]
         self.copies = []
         for _ in range(copies):
-            k = self.src_k.clone()
-            vs = [v.clone() for v in self.src_v] if value_count != -1 else \
-                [torch.empty_like(self.src_k)]
+            k = keys.clone()
+            vs = [v.clone() for v in values] if value_count != -1 else [torch.empty_like(keys)]
             self.copies.append((k, vs))
         torch.cuda.synchronize()
 
@@ -101,7 +105,7 @@ class Workload:
 
 
 def verify_on_device(torch, keys_in, keys_out, perm, first_bit, end_bit):
-    """The stable-sort properties on the GPU (torch): perm is a permutation,
+    """The stable-sort properties on the GPU: perm is a permutation,
     keys_out = keys_in[perm], bit field nondecreasing, ties in source order."""
     mask = (1 << (end_bit - first_bit)) - 1
     kin = keys_in.long() & 0xFFFFFFFF
@@ -121,8 +125,8 @@ def verify_on_device(torch, keys_in, keys_out, perm, first_bit, end_bit):
 
 
 def time_workload(torch, eng, wl, steps, warmup, timing=False):
-    """Wall time of `steps` sorts (barrier + sync on both sides); optional
-    per-kernel event timing of the same steps."""
+    """Wall time per sort of `steps` sorts (synchronized on both sides), and
+    optionally the per-kernel event timings of the same sorts."""
     for i in range(warmup):
         eng.sort_array(wl.data(i))
     torch.cuda.synchronize()
@@ -141,33 +145,31 @@ def time_workload(torch, eng, wl, steps, warmup, timing=False):
     return (t1 - t0) / steps, datas, kt
 
 
-def run_workload(torch, eng, name, keys_np, values_np, value_count, first_bit, end_bit, steps,
+def run_workload(torch, eng, name, keys, values, value_count, first_bit, end_bit, steps,
                  warmup, timing=False, verify=True):
     import mgpu_old_amd as mg
-    n = keys_np.size
+    n = keys.numel()
     log(f"[bench] {name}: n={n} vc={value_count} bits=[{first_bit},{end_bit}) "
         f"copies={steps + warmup}")
-    wl = Workload(torch, keys_np, values_np, value_count, first_bit, end_bit, steps + warmup)
+    wl = Workload(torch, keys, values, value_count, first_bit, end_bit, steps + warmup)
     sec, datas, kt = time_workload(torch, eng, wl, steps, warmup, timing)
     passes = len(mg.plan_passes(end_bit - first_bit))
+    v = 1 if value_count == -1 else value_count
     out = {
         "workload": name, "n": n, "value_count": value_count, "bits": [first_bit, end_bit],
         "passes": passes, "ms_per_sort": sec * 1e3, "gkeys_per_s": n / sec / 1e9,
     }
     b = sort_bytes(n, value_count, passes)
     out["sort_roofline"] = {"bytes_alg": b, "achieved_GBps": b / sec / 1e9,
-                            "frac": b / sec / HBM_PEAK,
-                            "floor_bytes": 8 * n * (1 + (1 if value_count == -1 else value_count)),
-                            }
+                            "frac": b / sec / HBM_PEAK, "floor_bytes": 8 * n * (1 + v),
+                            "floor_frac": 8 * n * (1 + v) / sec / HBM_PEAK}
     if verify:
         k_out, v_out = wl.result(datas[0])
         if value_count in (1, -1):
-            perm = v_out if value_count == -1 else v_out  # values hold 0..N-1 / indices
-            ok = verify_on_device(torch, wl.src_k, k_out, perm, first_bit, end_bit)
+            ok = verify_on_device(torch, wl.src_k, k_out, v_out, first_bit, end_bit)
         else:
-            kin = wl.src_k.long() & 0xFFFFFFFF
-            kout = k_out.long() & 0xFFFFFFFF
-            ok = bool((torch.sort(kin).values == kout).all())
+            ok = bool((torch.sort(wl.src_k.long() & 0xFFFFFFFF).values ==
+                       (k_out.long() & 0xFFFFFFFF)).all())
         out["bit_exact_properties"] = ok
         log(f"[bench] {name}: verify={'ok' if ok else 'FAILED'}")
     out["kernels"] = kt
@@ -175,6 +177,153 @@ def run_workload(torch, eng, name, keys_np, values_np, value_count, first_bit, e
     torch.cuda.empty_cache()
     log(f"[bench] {name}: {out['gkeys_per_s']:.2f} Gkeys/s, {out['ms_per_sort']:.3f} ms, "
         f"sort roofline {out['sort_roofline']['frac']:.3f}")
+    return out
+
+
+def segmented_bench(torch, eng, steps):
+    """4096 segments, sizes log-uniform in [1, 2^16], key + one value array:
+    graph-replayed segmented plan vs a loop of sortArray per segment."""
+    import mgpu_old_amd as mg
+    from mgpu_old_amd import workloads
+    sizes = workloads.log_uniform_sizes(4096, 0, 16, seed=4)
+    starts = np.zeros(len(sizes) + 1, dtype=np.int64)
+    starts[1:] = np.cumsum(sizes)
+    n = int(starts[-1])
+    log(f"[bench] segmented: 4096 segments, {n} keys")
+    keys = workloads.uniform_keys_torch(n, 44, "cuda", torch=torch)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    copies = []
+    for _ in range(steps + 1):
+        k, v = keys.clone(), vals.clone()
+        k1, v1 = torch.empty_like(k), torch.empty_like(v)
+        d = mg.SortData()
+        d.maxElements = d.numElements = n
+        d.valueCount, d.firstBit, d.endBit = 1, 0, 32
+        d.keys[0], d.keys[1] = k.data_ptr(), k1.data_ptr()
+        d.values[0][0], d.values[0][1] = v.data_ptr(), v1.data_ptr()
+        copies.append((d, k, v, k1, v1, eng.segmented_plan(d, starts)))
+    torch.cuda.synchronize()
+    copies[0][5].execute(copies[0][0])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for d, *_rest, plan in copies[1:]:
+        plan.execute(d)
+    torch.cuda.synchronize()
+    seg_s = (time.perf_counter() - t0) / steps
+    eng.check_device()
+    seg_k, seg_v = copies[1][1], copies[1][2]  # 4 passes: result back in side 0
+
+    # Baseline: one sortArray per segment on the same input.
+    k, v = keys.clone(), vals.clone()
+    k1, v1 = torch.empty_like(k), torch.empty_like(v)
+    datas = []
+    for i in range(len(sizes)):
+        a, b = int(starts[i]), int(starts[i + 1])
+        d = mg.SortData()
+        d.maxElements = d.numElements = b - a
+        d.valueCount, d.firstBit, d.endBit = 1, 0, 32
+        d.keys[0], d.keys[1] = k[a:].data_ptr(), k1[a:].data_ptr()
+        d.values[0][0], d.values[0][1] = v[a:].data_ptr(), v1[a:].data_ptr()
+        datas.append(d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for d in datas:
+        eng.sort_array(d)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    eng.check_device()
+    same = bool(torch.equal(seg_k, k)) and bool(torch.equal(seg_v, v))
+    for c in copies:
+        c[5].close()
+    out = {"segments": 4096, "n": n, "ms_segmented": seg_s * 1e3,
+           "gkeys_per_s_segmented": n / seg_s / 1e9, "ms_sortarray_loop": loop_s * 1e3,
+           "gkeys_per_s_sortarray_loop": n / loop_s / 1e9, "speedup_vs_loop": loop_s / seg_s,
+           "bit_exact_vs_loop": same}
+    log(f"[bench] segmented: {out['ms_segmented']:.3f} ms vs loop {out['ms_sortarray_loop']:.1f} ms "
+        f"= {out['speedup_vs_loop']:.1f}x, bit-exact={same}")
+    del copies
+    torch.cuda.empty_cache()
+    return out
+
+
+def served_bench(torch, eng, count):
+    """Poisson stream of sort requests through the sort server."""
+    import mgpu_old_amd as mg
+    from mgpu_old_amd import workloads
+    sizes = workloads.log_uniform_sizes(count, 10, 24, seed=5)
+    w = workloads.words(6, 0, 2 * count)
+    widths = np.array([16, 24, 32])[w[:count] % 3]
+    u = w[count:] % 100
+    vcs = np.where(u < 50, 0, np.where(u < 85, 1, 2))
+    total = int(sizes.sum())
+    log(f"[bench] served: {count} requests, {total} keys")
+    pristine, live, reqs = [], [], []
+    for i in range(count):
+        n, wd, vc = int(sizes[i]), int(widths[i]), int(vcs[i])
+        k = workloads.uniform_keys_torch(n, 10_000 + i, "cuda", mask_bits=wd, torch=torch)
+        vs = [workloads.uniform_keys_torch(n, 20_000 + 7 * i + j, "cuda", torch=torch)
+              for j in range(vc)]
+        pristine.append((k, vs))
+        lk, lvs = k.clone(), [x.clone() for x in vs]
+        live.append((lk, lvs))
+        r = mg.SortRequest()
+        r.keys = lk.data_ptr()
+        for j, x in enumerate(lvs):
+            r.values[j] = x.data_ptr()
+        r.numElements, r.valueCount, r.firstBit, r.endBit = n, vc, 0, wd
+        reqs.append(r)
+    budget = 16 << 30
+
+    def reset():
+        for (k, vs), (lk, lvs) in zip(pristine, live):
+            lk.copy_(k)
+            for a, b in zip(lvs, vs):
+                a.copy_(b)
+        torch.cuda.synchronize()
+
+    def run(arrivals):
+        reset()
+        with mg.Server(eng, budget, max_in_flight=3, max_elements=1 << 24) as srv:
+            comps = srv.replay(reqs, arrivals)
+        assert all(c.status == 0 for c in comps)
+        lat = np.array([c.doneMs - c.submitMs for c in comps])
+        span = max(c.doneMs for c in comps) - min(c.submitMs for c in comps)
+        return {"p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "gkeys_per_s": total / span / 1e6,
+                "batches": len({c.batch for c in comps}), "makespan_ms": span}
+
+    run(np.zeros(count))  # warm-up
+    cap = run(np.zeros(count))  # all at t=0: capacity
+    capacity = cap["gkeys_per_s"] * 1e9
+    mean_keys = total / count
+    out = {"requests": count, "keys": total, "hbm_budget_bytes": budget, "max_in_flight": 3,
+           "capacity": cap, "loads": {}}
+    for frac in (0.5, 0.8):
+        rate = frac * capacity / mean_keys  # requests per second
+        uu = (workloads.words(77, 0, count).astype(np.float64) + 0.5) / 4294967296.0
+        arrivals = np.cumsum(-np.log(uu) / rate * 1e3)
+        res = run(arrivals)
+        res["offered_gkeys_per_s"] = frac * capacity / 1e9
+        res["offered_requests_per_s"] = rate
+        out["loads"][f"{int(frac * 100)}%"] = res
+        log(f"[bench] served at {int(frac * 100)}% load: p50 {res['p50_ms']:.3f} ms, "
+            f"p99 {res['p99_ms']:.3f} ms, {res['gkeys_per_s']:.2f} Gkeys/s")
+    # Each served result equals the same request sorted alone by sortArray.
+    same = True
+    for i in range(0, count, max(1, count // 12)):
+        (k, vs), (lk, lvs) = pristine[i], live[i]
+        n, vc, wd = reqs[i].numElements, reqs[i].valueCount, reqs[i].endBit
+        wl = Workload(torch, k, vs, vc, 0, wd, 1)
+        d = wl.data(0)
+        eng.sort_array(d)
+        torch.cuda.synchronize()
+        ak, av = wl.result(d)
+        same = same and bool(torch.equal(ak, lk))
+        if vc:
+            same = same and bool(torch.equal(av, lvs[0]))
+    out["alone_equals_served"] = same
+    del pristine, live
+    torch.cuda.empty_cache()
     return out
 
 
@@ -192,14 +341,14 @@ def cpu_baseline(log2n):
     dt = time.perf_counter() - t0
     return {"value": n / dt / 1e9, "unit": "Gkeys/s", "cores": 1, "kind": "port",
             "sample": f"oracle std::stable_sort of 2^{log2n} uniform u32 keys + 1 u32 value "
-                      f"(same workload family as the headline, {dt:.1f} s)"}
+                      f"(the headline workload at 1/{1 << (28 - log2n)} size, {dt:.1f} s)"}
 
 
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
-                          "note only", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; how the sort would shard over "
+                          "several is a design note only", "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
     import mgpu_old_amd as mg
@@ -210,46 +359,48 @@ def main():
     eng.set_stream(torch.cuda.current_stream().cuda_stream)
 
     n = 1 << args.log2n
-    log(f"[bench] generating 2^{args.log2n} keys")
-    keys = workloads.uniform_keys(n, seed=2028)
-    vals = np.arange(n, dtype=np.uint32)
+    keys = workloads.uniform_keys_torch(n, 2028, "cuda", torch=torch)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
     head = run_workload(torch, eng, f"sortArray 2^{args.log2n} u32 key + 1 u32 value (0..N-1), "
                         "uniform keys, bits [0,32)", keys, [vals], 1, 0, 32, args.steps,
-                        args.warmup, timing=False)
-    # Same workload again with per-kernel events for the dominant kernel.
-    timed = run_workload(torch, eng, "timed", keys, [vals], 1, 0, 32, min(args.steps, 5), 1,
-                         timing=True, verify=False)
+                        args.warmup)
+    timed = run_workload(torch, eng, "per-kernel timing", keys, [vals], 1, 0, 32,
+                         min(args.steps, 5), 1, timing=True, verify=False)
     del keys, vals
     kt = timed["kernels"]
     scatter = {k: v for k, v in kt.items() if k.startswith("mgpu_rank_scatter")}
     name, (launches, total_ms, mn, mx) = max(scatter.items(), key=lambda kv: kv[1][1])
     avg_s = total_ms / launches / 1e3
-    per_launch = 8 * n * 2  # read + write of keys and one value array
+    per_launch = 8 * n * 2  # read + write of the keys and one value array
     roof = {"bound": "hbm", "kernel": name, "achieved": per_launch / avg_s / 1e9,
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per_launch / avg_s / HBM_PEAK,
             "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": launches,
             "bytes_per_launch": per_launch}
-    kernel_ms = {k: v[1] / v[0] for k, v in kt.items()}
+    kernel_ms = {k: {"avg_ms": v[1] / v[0], "launches": v[0]} for k, v in kt.items()}
 
     extra = []
     if not args.no_extra:
-        k1 = workloads.uniform_keys(1 << 20, seed=1)
+        k1 = workloads.uniform_keys_torch(1 << 20, 1, "cuda", torch=torch)
         extra.append(run_workload(torch, eng, "2^20 u32 keys only, [0,32)", k1, [], 0, 0, 32,
                                   20, 3))
-        k2 = workloads.uniform_keys(1 << 25, seed=2)
-        extra.append(run_workload(torch, eng, "2^25 u32 key + value, [0,32)", k2,
-                                  [np.arange(1 << 25, dtype=np.uint32)], 1, 0, 32, 10, 3))
-        del k1, k2
-        log("[bench] generating 2^28 low-entropy keys")
-        k3 = workloads.and3_keys(1 << 28, seed=3)
+        k2 = workloads.uniform_keys_torch(1 << 25, 2, "cuda", torch=torch)
+        v2 = torch.arange(1 << 25, dtype=torch.int32, device="cuda")
+        extra.append(run_workload(torch, eng, "2^25 u32 key + value (0..N-1), [0,32)", k2, [v2],
+                                  1, 0, 32, 10, 3))
+        del k1, k2, v2
+        k3 = workloads.and3_keys_torch(1 << 28, 3, "cuda", torch=torch)
         extra.append(run_workload(torch, eng, "2^28 AND3 keys, index generation, [0,32)", k3,
                                   [], -1, 0, 32, 5, 2))
-        k3 &= np.uint32(0xFFFFFF)
+        k3 &= 0xFFFFFF
         extra.append(run_workload(torch, eng, "2^28 AND3 keys & 0xFFFFFF, index generation, "
                                   "[0,24)", k3, [], -1, 0, 24, 5, 2))
         del k3
+        torch.cuda.empty_cache()
     for e in extra:
         e.pop("kernels", None)
+
+    seg = None if args.no_segmented else segmented_bench(torch, eng, 10)
+    served = None if args.no_served else served_bench(torch, eng, args.served_requests)
 
     cpu = None
     if not args.no_cpu:
@@ -257,7 +408,6 @@ def main():
         cpu = cpu_baseline(args.cpu_log2n)
         cpu["cores_available"] = os.cpu_count()
 
-    ms = head["ms_per_sort"]
     line = {
         "metric": "Gkeys/s, 2^28 u32 key+value sort; %HBM roofline; batched/served p99 latency",
         "value": head["gkeys_per_s"],
@@ -265,19 +415,22 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": head["ms_per_sort"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic (SplitMix64-seeded uniform keys, values 0..N-1)",
         "config": {"workload": head["workload"], "n": n, "value_count": 1, "bits": [0, 32],
-                   "passes": head["passes"], "bit_exact_properties": head["bit_exact_properties"]},
+                   "passes": head["passes"],
+                   "bit_exact_properties": head["bit_exact_properties"]},
         "roofline": roof,
         "sort_roofline": head["sort_roofline"],
         "kernel_avg_ms": kernel_ms,
         "cpu_baseline": cpu,
         "extra": extra,
+        "segmented": seg,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
     eng.close()

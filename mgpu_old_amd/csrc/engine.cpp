@@ -15,6 +15,7 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -78,7 +79,7 @@ sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, i
 }
 
 sortStatus_t mgpu_lane_init(SortLane* l) {
-    const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_MAX_PASSES;
+    const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_LANE_COUNTERS;
     if (hipMalloc(&l->fixed, words * sizeof(uint32_t)) != hipSuccess) {
         l->fixed = nullptr;
         return SORT_STATUS_DEVICE_ALLOC_FAILED;
@@ -155,7 +156,10 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
 sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firstBit,
                               int numPasses, const int* widths, int valueCount,
                               uint32_t* const keyBuf[2],
-                              uint32_t* const valBuf[MGPU_MAX_VALUES][2]) {
+                              uint32_t* const valBuf[MGPU_MAX_VALUES][2], bool earlyExit,
+                              int* executed) {
+    int done = 0;
+    if (executed) *executed = 0;
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     if (numPasses > MGPU_MAX_PASSES) return SORT_STATUS_INTERNAL_ERROR;
     const SortConfig& cfg = e->config;
@@ -168,7 +172,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     if (L.histDirty) {
         if (hipMemsetAsync(L.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
                            L.stream) != hipSuccess ||
-            hipMemsetAsync(L.tileCounters, 0, MGPU_MAX_PASSES * sizeof(uint32_t), L.stream) !=
+            hipMemsetAsync(L.tileCounters, 0, MGPU_LANE_COUNTERS * sizeof(uint32_t), L.stream) !=
                 hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
         L.histDirty = false;
@@ -184,13 +188,20 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     ca.n = n;
     ca.num_passes = (uint32_t)numPasses;
     ca.vec_ok = (reinterpret_cast<uintptr_t>(keyBuf[0]) & 15) == 0;
+    int totalBits = 0;
     {
         int bit = firstBit;
         for (int p = 0; p < numPasses; ++p) {
             ca.shift[p] = (uint32_t)bit;
             ca.mask[p] = (1u << widths[p]) - 1u;
             bit += widths[p];
+            totalBits += widths[p];
         }
+    }
+    if (earlyExit) {
+        ca.unsorted = L.tileCounters + MGPU_UNSORTED_SLOT;
+        ca.check_shift = (uint32_t)firstBit;
+        ca.check_mask = totalBits >= 32 ? 0xffffffffu : ((1u << totalBits) - 1u);
     }
     uint32_t grid = (n + 8191) / 8192;
     uint32_t maxGrid = (uint32_t)e->numCUs * 4;
@@ -207,7 +218,30 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     st = mgpu_launch(e, L.stream, e->fnCount, grid, 256, &ca);
     if (st) return st;
 
-    // digit_scan
+    // Early exit (the role of the reference's sortDetectCounters readback,
+    // mgpusort.cpp:433-451, for all passes at once): a pass whose digit is the
+    // same for every key is the identity and is skipped; already-sorted input
+    // skips every pass.
+    bool run[MGPU_MAX_PASSES];
+    for (int p = 0; p < numPasses; ++p) run[p] = true;
+    bool sorted = false;
+    if (earlyExit) {
+        std::vector<uint32_t> h((size_t)MGPU_MAX_PASSES * MGPU_RADIX + MGPU_LANE_COUNTERS);
+        if (hipMemcpyAsync(h.data(), L.hist, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
+            hipMemcpyAsync(h.data() + MGPU_MAX_PASSES * MGPU_RADIX, L.tileCounters,
+                           MGPU_LANE_COUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           L.stream) != hipSuccess ||
+            hipStreamSynchronize(L.stream) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        sorted = h[MGPU_MAX_PASSES * MGPU_RADIX + MGPU_UNSORTED_SLOT] == 0;
+        for (int p = 0; p < numPasses; ++p) {
+            const uint32_t* hp = h.data() + p * MGPU_RADIX;
+            run[p] = !sorted && *std::max_element(hp, hp + MGPU_RADIX) != n;
+        }
+    }
+
+    // digit_scan (also puts hist and the counters back to zero)
     mgpuScanArgs sa;
     sa.hist = L.hist;
     sa.digit_start = L.digitStart;
@@ -217,17 +251,20 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     if (st) return st;
     L.histDirty = false;
 
-    // rank_scatter per pass
+    // rank_scatter per executed pass; executed pass k reads side k & 1.
     int bit = firstBit;
     for (int p = 0; p < numPasses; ++p) {
+        const int shiftBits = bit;
+        bit += widths[p];
+        if (!run[p]) continue;
         mgpuPassArgs pa;
         std::memset(&pa, 0, sizeof(pa));
-        const int in = p & 1, out = in ^ 1;
+        const int in = done & 1, out = in ^ 1;
         pa.keys_in = keyBuf[in];
         pa.keys_out = keyBuf[out];
         int mode;
         if (valueCount == 0) mode = MGPU_VALUES_NONE;
-        else if (valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
+        else if (valueCount == -1) mode = done == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
         else if (valueCount == 1) mode = MGPU_VALUES_SINGLE;
         else mode = MGPU_VALUES_MULTI;
         int carried = valueCount == -1 ? 1 : valueCount;
@@ -241,12 +278,23 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         pa.error_flag = e->errorFlag;
         pa.n = n;
         pa.num_tiles = numTiles;
-        pa.shift = (uint32_t)bit;
+        pa.shift = (uint32_t)shiftBits;
         pa.mask = (1u << widths[p]) - 1u;
         pa.num_values = (uint32_t)carried;
         st = mgpu_launch(e, L.stream, e->fnPass[mode], numTiles, cfg.blockThreads, &pa);
         if (st) return st;
-        bit += widths[p];
+        ++done;
+        if (executed) *executed = done;
+    }
+    // Index generation with no pass left to run: the indices are the identity.
+    if (done == 0 && valueCount == -1) {
+        mgpuIotaArgs ia;
+        ia.out = valBuf[0][0];
+        ia.n = n;
+        ia.pad_ = 0;
+        uint32_t g = std::min<uint32_t>((n + 255) / 256, (uint32_t)e->numCUs * 8);
+        st = mgpu_launch(e, L.stream, e->fnIota, g, 256, &ia);
+        if (st) return st;
     }
     return SORT_STATUS_SUCCESS;
 }
@@ -287,14 +335,16 @@ sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, co
     for (int v = 0; v < MGPU_MAX_VALUES; ++v)
         for (int s = 0; s < 2; ++s) valBuf[v][s] = reinterpret_cast<uint32_t*>(data->values[v][s]);
 
+    int executed = 0;
     sortStatus_t st = mgpu_sort_passes(e, e->lane, n, data->firstBit, numPasses, widths,
-                                       data->valueCount, keyBuf, valBuf);
+                                       data->valueCount, keyBuf, valBuf, data->earlyExit,
+                                       &executed);
     if (st) return st;
-    if (numPasses & 1) {
+    if (executed & 1) {
         std::swap(data->keys[0], data->keys[1]);
         for (int v = 0; v < MGPU_MAX_VALUES; ++v) std::swap(data->values[v][0], data->values[v][1]);
     }
-    data->parity ^= (numPasses & 1);
+    data->parity ^= (executed & 1);
     return SORT_STATUS_SUCCESS;
 }
 

@@ -12,10 +12,12 @@
 #include "kernel_args.h"
 #include "segmented_internal.h"
 
-// Tile shape of the rank_scatter kernel: blockThreads * itemsPerThread keys.
+// Variant of the rank_scatter kernel: tile of blockThreads * itemsPerThread
+// keys; in-wave ranking by LDS match masks (0) or per-bit ballots (1).
 struct SortConfig {
     uint32_t blockThreads;
     uint32_t itemsPerThread;
+    uint32_t rankMethod;
 };
 
 // One in-order queue of sorts: a stream plus the device scratch of the
@@ -23,7 +25,7 @@ struct SortConfig {
 // chosen with sortSetStream); the sort server owns one per in-flight batch.
 struct SortLane {
     hipStream_t stream = nullptr;
-    uint32_t* fixed = nullptr;  // hist[4][256], digitStart[4][256], tileCounters[4]
+    uint32_t* fixed = nullptr;  // hist[4][256], digitStart[4][256], counters[8]
     uint32_t* hist = nullptr;
     uint32_t* digitStart = nullptr;
     uint32_t* tileCounters = nullptr;
@@ -45,8 +47,9 @@ struct sortEngine_d {
     hipModule_t radixModule = nullptr;
     hipFunction_t fnCount = nullptr;
     hipFunction_t fnScan = nullptr;
+    hipFunction_t fnIota = nullptr;
     hipFunction_t fnPass[4] = {};  // by value mode (MGPU_VALUES_*)
-    SortConfig config = {256, 16};
+    SortConfig config = {256, 16, 0};
 
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;
@@ -85,10 +88,14 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
 // `passes` passes (a reallocation synchronizes the lane's stream).
 sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes);
 
-// Runs numPasses digit passes over n keys starting at firstBit.  Pass p reads
-// buffer side (p & 1) and writes side (p & 1) ^ 1 of keyBuf / valBuf, so the
-// result is on side (numPasses & 1).  valueCount as in sortData_d.
+// Runs numPasses digit passes over n keys starting at firstBit.  Executed pass
+// k reads buffer side (k & 1) and writes side (k & 1) ^ 1 of keyBuf / valBuf,
+// so the result is on side (executed & 1).  Without earlyExit every pass
+// runs; with it, passes whose digit is constant are skipped, and all of them
+// when the keys are already in order (one blocking readback).  valueCount as
+// in sortData_d.
 sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& lane, uint32_t n, int firstBit,
-                              int numPasses,
-                              const int* widths, int valueCount, uint32_t* const keyBuf[2],
-                              uint32_t* const valBuf[MGPU_MAX_VALUES][2]);
+                              int numPasses, const int* widths, int valueCount,
+                              uint32_t* const keyBuf[2],
+                              uint32_t* const valBuf[MGPU_MAX_VALUES][2],
+                              bool earlyExit = false, int* executed = nullptr);

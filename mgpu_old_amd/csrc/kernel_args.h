@@ -34,6 +34,11 @@ struct mgpuCountArgs {
     uint32_t vec_ok;         // keys pointer is 16-byte aligned
     uint32_t shift[MGPU_MAX_PASSES];
     uint32_t mask[MGPU_MAX_PASSES];
+    // Sortedness check (early exit): counts adjacent pairs whose bit field
+    // (k >> check_shift) & check_mask decreases into *unsorted.
+    uint32_t* unsorted;      // null: no check
+    uint32_t check_shift;
+    uint32_t check_mask;
 };
 
 // digit_scan: exclusive scan of each pass's histogram into digit starts; puts
@@ -41,8 +46,20 @@ struct mgpuCountArgs {
 struct mgpuScanArgs {
     uint32_t* hist;           // [MGPU_MAX_PASSES][MGPU_RADIX] -> zeroed
     uint32_t* digit_start;    // [MGPU_MAX_PASSES][MGPU_RADIX]
-    uint32_t* tile_counters;  // [MGPU_MAX_PASSES] -> zeroed
+    uint32_t* tile_counters;  // [MGPU_LANE_COUNTERS] -> zeroed (tile counters, unsorted count)
     uint32_t num_passes;
+};
+
+// Words after digitStart in a lane's scratch: one tile counter per pass, then
+// the unsorted-pair count of the early-exit check.
+#define MGPU_LANE_COUNTERS 8
+#define MGPU_UNSORTED_SLOT MGPU_MAX_PASSES
+
+// iota: out[i] = i for i < n (index output of a sort that needs no pass).
+struct mgpuIotaArgs {
+    uint32_t* out;
+    uint32_t n;
+    uint32_t pad_;
 };
 
 // ---------------------------------------------------------------------------

@@ -20,15 +20,21 @@ sortStatus_t get_fn(sortEngine_d* e, hipModule_t m, const std::string& name, hip
     return SORT_STATUS_SUCCESS;
 }
 
-// Tile shape: MGPUSORT_TILE="<threads>x<items>" picks one of the built
-// shapes (for sweeps); otherwise the default.
+// Tile shape and ranking method: MGPUSORT_TILE="<threads>x<items>" and
+// MGPUSORT_RANK=<0|1> pick one of the built variants (for sweeps); otherwise
+// the default.
 SortConfig pick_config() {
-    SortConfig c = {256, 16};
+    SortConfig c = {256, 16, 0};
     const char* t = std::getenv("MGPUSORT_TILE");
     if (t && *t) {
         unsigned b = 0, i = 0;
-        if (std::sscanf(t, "%ux%u", &b, &i) == 2 && b && i) c = {b, i};
+        if (std::sscanf(t, "%ux%u", &b, &i) == 2 && b && i) {
+            c.blockThreads = b;
+            c.itemsPerThread = i;
+        }
     }
+    const char* r = std::getenv("MGPUSORT_RANK");
+    if (r && (*r == '0' || *r == '1')) c.rankMethod = (uint32_t)(*r - '0');
     return c;
 }
 
@@ -42,11 +48,12 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     }
     sortStatus_t st = get_fn(e, e->radixModule, "mgpu_digit_count", &e->fnCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
     e->config = pick_config();
     for (int v = 0; v < 4 && !st; ++v) {
         char name[96];
-        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_v%d", e->config.blockThreads,
-                      e->config.itemsPerThread, v);
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u_v%d",
+                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod, v);
         st = get_fn(e, e->radixModule, name, &e->fnPass[v]);
     }
     if (st) return st;

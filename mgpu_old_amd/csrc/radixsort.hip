@@ -22,16 +22,12 @@
 // its 64 keys against each other by digit with a match mask (the lanes that
 // hold the same digit), the rank being the number of matching lower lanes
 // plus the wave's running count of that digit.  The match mask is built
-// with LDS atomic ORs (default) or, with MGPU_RANK_BALLOT=1, with one ballot
-// per digit bit.
+// with LDS atomic ORs (RANK 0) or with one ballot per digit bit (RANK 1); both
+// are built, the engine picks one.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "kernel_args.h"
-
-#ifndef MGPU_RANK_BALLOT
-#define MGPU_RANK_BALLOT 0
-#endif
 
 namespace {
 
@@ -115,6 +111,12 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
 
     const uint32_t begin = blockIdx.x * a.chunk;
     const uint32_t end = min(a.n, begin + a.chunk);
+    // Early-exit check: descents of the sort's bit field between neighbours
+    // (each key against the one after it, across chunk ends too).
+    const bool check = a.unsorted != nullptr;
+    const uint32_t cs = a.check_shift, cm = a.check_mask;
+    uint32_t descents = 0;
+    auto bf = [&](uint32_t k) { return (k >> cs) & cm; };
     if (begin < end) {
         if (a.vec_ok) {
             const uint4* v = reinterpret_cast<const uint4*>(a.keys + begin);
@@ -125,12 +127,27 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                 count_key(q.y);
                 count_key(q.z);
                 count_key(q.w);
+                if (check) {
+                    descents += (bf(q.x) > bf(q.y)) + (bf(q.y) > bf(q.z)) + (bf(q.z) > bf(q.w));
+                    const uint32_t next = begin + 4 * i + 4;
+                    if (next < a.n) descents += bf(q.w) > bf(a.keys[next]);
+                }
             }
-            for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) count_key(a.keys[i]);
+            for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) {
+                const uint32_t k = a.keys[i];
+                count_key(k);
+                if (check && i + 1 < a.n) descents += bf(k) > bf(a.keys[i + 1]);
+            }
         } else {
-            for (uint32_t i = begin + tid; i < end; i += BLOCK) count_key(a.keys[i]);
+            for (uint32_t i = begin + tid; i < end; i += BLOCK) {
+                const uint32_t k = a.keys[i];
+                count_key(k);
+                if (check && i + 1 < a.n) descents += bf(k) > bf(a.keys[i + 1]);
+            }
         }
     }
+    if (check && descents)
+        __hip_atomic_fetch_add(a.unsorted, descents, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
 
     for (uint32_t i = tid; i < (uint32_t)(MGPU_MAX_PASSES * RADIX); i += BLOCK) {
@@ -153,12 +170,12 @@ __device__ void digit_scan_body(const mgpuScanArgs& a) {
         a.digit_start[p * RADIX + tid] = e;
         a.hist[p * RADIX + tid] = 0;
     }
-    if (tid < MGPU_MAX_PASSES) a.tile_counters[tid] = 0;
+    if (tid < MGPU_LANE_COUNTERS) a.tile_counters[tid] = 0;
 }
 
 // ---------------------------------------------------------------------------
 // rank_scatter: one digit pass over one tile of BLOCK*IPT keys per workgroup.
-template <int BLOCK, int IPT, int VMODE>
+template <int BLOCK, int IPT, int RANK, int VMODE>
 __device__ void rank_scatter_body(const mgpuPassArgs& a) {
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
@@ -167,9 +184,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
 
     __shared__ uint32_t s_stage[TILE];
     __shared__ uint32_t s_count[W][RADIX];
-#if !MGPU_RANK_BALLOT
-    __shared__ uint64_t s_match[W][RADIX];
-#endif
+    __shared__ uint64_t s_match[RANK == 0 ? W : 1][RANK == 0 ? RADIX : 1];
     __shared__ uint32_t s_global[RADIX];
     __shared__ uint32_t s_tmp[W];
     __shared__ uint32_t s_tile;
@@ -183,9 +198,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
                                         __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
         (&s_count[0][0])[i] = 0;
-#if !MGPU_RANK_BALLOT
-        (&s_match[0][0])[i] = 0;
-#endif
+        if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
     }
     __syncthreads();
     const uint32_t tile = s_tile;
@@ -240,28 +253,28 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
     for (int j = 0; j < IPT; ++j) {
         const uint32_t d = digit_of(j);
         uint64_t m;
-#if MGPU_RANK_BALLOT
-        m = ~0ull;
+        if constexpr (RANK == 1) {
+            m = ~0ull;
 #pragma unroll
-        for (int b = 0; b < MGPU_RADIX_BITS; ++b) {
-            const uint32_t bit = (d >> b) & 1u;
-            const uint64_t bb = __builtin_amdgcn_ballot_w64(bit != 0);
-            m &= bit ? bb : ~bb;
+            for (int b = 0; b < MGPU_RADIX_BITS; ++b) {
+                const uint32_t bit = (d >> b) & 1u;
+                const uint64_t bb = __builtin_amdgcn_ballot_w64(bit != 0);
+                m &= bit ? bb : ~bb;
+            }
+        } else {
+            __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
         }
-#else
-        __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_WORKGROUP);
-        m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-#endif
         const uint32_t below = lanes_below(m);
         const uint32_t old = s_count[wave][d];
         rank[j] = old + below;
         if (below == 0) {  // lowest lane of the group advances the count
             s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
-#if !MGPU_RANK_BALLOT
-            __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
-#endif
+            if constexpr (RANK == 0)
+                __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
     __syncthreads();
@@ -381,24 +394,24 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_digit_scan(mgpuScanArgs a
     digit_scan_body(a);
 }
 
-#define MGPU_RANK_SCATTER(B, I)                                                               \
-    extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_v0(mgpuPassArgs a) {                                  \
-        rank_scatter_body<B, I, MGPU_VALUES_NONE>(a);                                         \
-    }                                                                                         \
-    extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_v1(mgpuPassArgs a) {                                  \
-        rank_scatter_body<B, I, MGPU_VALUES_INDEX>(a);                                        \
-    }                                                                                         \
-    extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_v2(mgpuPassArgs a) {                                  \
-        rank_scatter_body<B, I, MGPU_VALUES_SINGLE>(a);                                       \
-    }                                                                                         \
-    extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_v3(mgpuPassArgs a) {                                  \
-        rank_scatter_body<B, I, MGPU_VALUES_MULTI>(a);                                        \
-    }
+extern "C" __global__ void __launch_bounds__(256) mgpu_iota(mgpuIotaArgs a) {
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += gridDim.x * 256) a.out[i] = i;
+}
 
-MGPU_RANK_SCATTER(256, 16)
-MGPU_RANK_SCATTER(256, 24)
-MGPU_RANK_SCATTER(512, 16)
+#define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
+        rank_scatter_body<B, I, R, MODE>(a);                                                  \
+    }
+#define MGPU_RANK_SCATTER(B, I, R)                                                            \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 1, MGPU_VALUES_INDEX)                                     \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 2, MGPU_VALUES_SINGLE)                                    \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
+
+MGPU_RANK_SCATTER(256, 16, 0)
+MGPU_RANK_SCATTER(256, 16, 1)
+MGPU_RANK_SCATTER(256, 24, 0)
+MGPU_RANK_SCATTER(256, 24, 1)
+MGPU_RANK_SCATTER(512, 16, 0)
+MGPU_RANK_SCATTER(512, 16, 1)

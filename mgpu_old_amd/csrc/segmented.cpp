@@ -383,7 +383,7 @@ sortStatus_t build_orientation(sortSegPlan_d* p, int o) {
     if (lane.histDirty) {  // no memset nodes inside the graph
         if (hipMemset(lane.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t)) !=
                 hipSuccess ||
-            hipMemset(lane.tileCounters, 0, MGPU_MAX_PASSES * sizeof(uint32_t)) != hipSuccess)
+            hipMemset(lane.tileCounters, 0, MGPU_LANE_COUNTERS * sizeof(uint32_t)) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
         lane.histDirty = false;
     }

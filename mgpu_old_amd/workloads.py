@@ -54,6 +54,57 @@ def and3_keys(n, seed, mask_bits=32, out=None):
     return out
 
 
+def _i64(x):
+    """uint64 constant as the int64 with the same bits."""
+    return x - (1 << 64) if x >= 1 << 63 else x
+
+
+def _torch_mix(z, torch):
+    """SplitMix64's mix on int64 tensors (two's-complement wraparound; logical
+    shifts done as arithmetic shift + mask)."""
+    def srl(x, k):
+        return (x >> k) & ((1 << (64 - k)) - 1)
+    z = (z ^ srl(z, 30)) * _i64(0xBF58476D1CE4E5B9)
+    z = (z ^ srl(z, 27)) * _i64(0x94D049BB133111EB)
+    return z ^ srl(z, 31)
+
+
+def words_torch(seed, start, count, device, torch=None):
+    """words(seed, start, count) computed on a torch device (int32 tensor with
+    the same bits as the uint32 words)."""
+    if torch is None:
+        import torch
+    i = torch.arange(start + 1, start + 1 + count, dtype=torch.int64, device=device)
+    z = _torch_mix(_i64(seed % (1 << 64)) + i * _i64(0x9E3779B97F4A7C15), torch)
+    # High 32 bits; the int32 cast keeps the low 32 bits of the shifted value.
+    return (z >> 32).to(torch.int32)
+
+
+def uniform_keys_torch(n, seed, device, mask_bits=32, torch=None):
+    """uniform_keys(n, seed, mask_bits) generated on the device (int32 bits)."""
+    if torch is None:
+        import torch
+    out = torch.empty(n, dtype=torch.int32, device=device)
+    mask = -1 if mask_bits >= 32 else (1 << mask_bits) - 1
+    for s in range(0, n, 1 << 26):
+        c = min(1 << 26, n - s)
+        out[s:s + c] = words_torch(seed, s, c, device, torch) & mask
+    return out
+
+
+def and3_keys_torch(n, seed, device, mask_bits=32, torch=None):
+    """and3_keys(n, seed, mask_bits) generated on the device (int32 bits)."""
+    if torch is None:
+        import torch
+    out = torch.empty(n, dtype=torch.int32, device=device)
+    mask = -1 if mask_bits >= 32 else (1 << mask_bits) - 1
+    for s in range(0, n, 1 << 24):
+        c = min(1 << 24, n - s)
+        w = words_torch(seed, 3 * s, 3 * c, device, torch).view(c, 3)
+        out[s:s + c] = w[:, 0] & w[:, 1] & w[:, 2] & mask
+    return out
+
+
 def log_uniform_sizes(count, lo_log2, hi_log2, seed, lo=None, hi=None):
     """count sizes floor(2 ** (lo_log2 + (hi_log2 - lo_log2) * u)), clamped to
     [lo, hi] (default [2**lo_log2, 2**hi_log2])."""

@@ -86,6 +86,45 @@ int main(void) {
     assert out == ["144", "20", "24", "32", "48", "128", "48", "-1"]
 
 
+def test_cpp_wrapper_compiles_links_and_throws_without_device(tmp_path):
+    """inc/mgpusort.hpp's classes (MgpuSort, MgpuSortData, MgpuException)
+    build against the library; with no device the engine constructor throws."""
+    src = tmp_path / "wrap.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include <cstring>
+#include "mgpusort.hpp"
+int main() {
+    MgpuSortData d;
+    unsigned int* fake = reinterpret_cast<unsigned int*>(0x1000);
+    d.AttachKey(fake);
+    d.AttachVal(0, fake + 64);
+    bool ok = d.pKeys[0] == fake && d.values1[0] == (CUdeviceptr)(fake + 64);
+    d.Reset();  // attached buffers are not freed
+    ok = ok && d.keys[0] == 0 && d.values1[0] == 0;
+    try {
+        MgpuSort engine(static_cast<const char*>(nullptr));
+        std::printf("engine %d\n", ok ? 1 : 0);
+    } catch (const MgpuException& e) {
+        std::printf("%s %d\n", e.what(), ok ? 1 : 0);
+    }
+    MgpuSort none(static_cast<sortEngine_t>(nullptr));
+    MgpuSort copy(none);
+    copy = none;
+    return 0;
+}
+''')
+    exe = tmp_path / "wrap"
+    lib_dir = mg.LIB_DIR
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L", lib_dir, f"-Wl,-rpath,{lib_dir}", "-lmgpusort"],
+                   check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    assert out[-1] == "1"
+    if not _gpu_visible():
+        assert out[0] == "SORT_STATUS_INVALID_CONTEXT"
+
+
 def test_status_strings_match_reference():
     for code, name in enumerate(REFERENCE_STATUS):
         assert mg.status_string(code) == name
@@ -154,4 +193,4 @@ def test_code_objects_built_for_gfx950():
         data = open(path, "rb").read()
         assert data[:4] == b"\x7fELF"
         assert b"gfx950" in data
-        assert b"mgpu_digit_count" in data and b"mgpu_rank_scatter_b256_i16_v0" in data
+        assert b"mgpu_digit_count" in data and b"mgpu_rank_scatter_b256_i16_r0_v0" in data

@@ -225,6 +225,33 @@ def test_sort_host_and_device(engine, oracle):
     assert np.array_equal(to_host(dk, n), rk32)
 
 
+@pytest.mark.parametrize("kind,executed", [("random", 4), ("sorted", 0), ("top_constant", 3),
+                                           ("low_constant", 3), ("two_constant", 2)])
+@pytest.mark.parametrize("vc", [-1, 1])
+def test_early_exit(engine, oracle, kind, executed, vc):
+    """earlyExit skips passes whose digit is the same for every key, and every
+    pass of already-sorted input; the output is unchanged and parity follows
+    the executed passes."""
+    n = 150_001
+    keys = workloads.uniform_keys(n, seed=90)
+    if kind == "sorted":
+        keys = np.sort(keys)
+    elif kind == "top_constant":
+        keys = (keys & np.uint32(0xFFFFFF)) | np.uint32(0x5A000000)
+    elif kind == "low_constant":
+        keys = (keys & np.uint32(0xFFFFFF00)) | np.uint32(0x33)
+    elif kind == "two_constant":
+        keys = (keys & np.uint32(0x00FFFF00)) | np.uint32(0x77000011)
+    vals = [np.arange(n, dtype=np.uint32)] if vc == 1 else []
+    g = GpuSort(keys, vals, value_count=vc)
+    g.data.earlyExit = True
+    g.run(engine)
+    engine.check_device()
+    _check_vs_oracle(oracle, keys, g, vals, vc)
+    assert g.data.parity == executed % 2
+    assert g.data.keys[g.data.parity] == g.start_keys
+
+
 def test_keys_past_the_end_are_untouched(engine):
     n, cap = 5000, 8192
     keys = workloads.uniform_keys(n, 4)

@@ -33,6 +33,17 @@ def test_workload_generators_agree(oracle, seed):
     assert [int(x) for x in w] == [oracle.L.oracle_splitmix64_at(seed, i) for i in (10, 11, 12)]
 
 
+@pytest.mark.parametrize("seed", [0, 77, 2**63 + 11])
+def test_torch_generators_agree(oracle, seed):
+    """The on-device (torch int64) generators give the oracle's words."""
+    torch = pytest.importorskip("torch")
+    n = 20000
+    a = workloads.uniform_keys_torch(n, seed, "cpu", torch=torch).numpy().view(np.uint32)
+    assert np.array_equal(a, oracle.fill_keys(n, seed, 0))
+    b = workloads.and3_keys_torch(n, seed, "cpu", 24, torch=torch).numpy().view(np.uint32)
+    assert np.array_equal(b, oracle.fill_keys(n, seed, 1, 24))
+
+
 def test_and3_keys_bit_density():
     k = workloads.and3_keys(1 << 16, 99)
     bits = np.unpackbits(k.view(np.uint8)).mean()
