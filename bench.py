@@ -44,6 +44,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log2n", type=int, default=25)
     ap.add_argument("--served-requests", type=int, default=600)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run one plain device copy of this size (calibrates the HBM "
+                         "byte counters of a profiler run)")
     return ap.parse_args()
 
 
@@ -357,6 +360,15 @@ def main():
     torch.cuda.set_device(0)
     eng = mg.Engine()
     eng.set_stream(torch.cuda.current_stream().cuda_stream)
+
+    if args.copy_probe_mib:
+        src = torch.empty(args.copy_probe_mib << 18, dtype=torch.int32, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
 
     n = 1 << args.log2n
     keys = workloads.uniform_keys_torch(n, 2028, "cuda", torch=torch)

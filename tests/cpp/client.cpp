@@ -1,0 +1,173 @@
+// A C++ client of the mgpusort C API written the way the reference's own
+// callers use it, built with g++ against include/ and libmgpusort.so only:
+//   1. simpletest2-style (sort/src/simpletest2/simpletest2.cpp:57-150):
+//      caller buffers attached through MgpuSortData, bits [7, 25), two value
+//      streams (uint and float), then index generation with earlyExit;
+//   2. bwt-style (bwt/src/mgpubwt/mgpubwt.cpp:262-292): multi-word keys
+//      sorted least significant word first by repeated sortArray calls that
+//      carry the remaining words plus an index stream as values, with the
+//      caller tracking buffers through `parity`.
+// Every result is checked against std::stable_sort on the host.  Prints
+// "CLIENT OK" on success and exits non-zero on the first mismatch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "mgpusort.hpp"
+
+#define CHECK(x)                                                              \
+    do {                                                                      \
+        if (!(x)) {                                                           \
+            std::printf("CHECK failed at line %d: %s\n", __LINE__, #x);      \
+            return 1;                                                         \
+        }                                                                     \
+    } while (0)
+
+static uint32_t* dalloc(size_t n) {
+    void* p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * 4) != hipSuccess) return nullptr;
+    return static_cast<uint32_t*>(p);
+}
+
+static int simple_test(MgpuSort& engine) {
+    const int n = 100000;
+    std::mt19937 rng(7);
+    std::vector<uint32_t> keys(n), v1(n);
+    std::vector<float> v2(n);
+    for (int i = 0; i < n; ++i) {
+        keys[i] = (rng() & ((1u << 18) - 1)) << 7;
+        v1[i] = i;
+        v2[i] = std::sqrt((float)i);
+    }
+    uint32_t *dk = dalloc(n), *dv1 = dalloc(n), *dv2 = dalloc(n);
+    CHECK(dk && dv1 && dv2);
+    CHECK(hipMemcpy(dk, keys.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+    CHECK(hipMemcpy(dv1, v1.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+    CHECK(hipMemcpy(dv2, v2.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+
+    MgpuSortData data;
+    data.AttachKey(dk);
+    data.AttachVal(0, dv1);
+    data.AttachVal(1, dv2);
+    CHECK(data.Alloc(engine, n, 2) == SORT_STATUS_SUCCESS);
+    data.firstBit = 7;
+    data.endBit = 7 + 18;
+    CHECK(sortArray(engine, &data) == SORT_STATUS_SUCCESS);
+
+    std::vector<uint32_t> ok(n), o1(n);
+    std::vector<float> o2(n);
+    CHECK(hipMemcpy(ok.data(), data.pKeys[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(o1.data(), data.pValues1[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(o2.data(), data.pValues2[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    auto bf = [](uint32_t k) { return (k >> 7) & ((1u << 18) - 1); };
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int a, int b) { return bf(keys[a]) < bf(keys[b]); });
+    for (int i = 0; i < n; ++i) {
+        CHECK(ok[i] == keys[perm[i]]);
+        CHECK(o1[i] == v1[perm[i]]);
+        CHECK(std::memcmp(&o2[i], &v2[perm[i]], 4) == 0);
+    }
+    data.Reset();  // frees only the library's buffers
+
+    // Index generation with early exit over the full 32 bits.
+    CHECK(hipMemcpy(dk, keys.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+    data.AttachKey(dk);
+    CHECK(data.Alloc(engine, n, SORT_VALUE_INDEX) == SORT_STATUS_SUCCESS);
+    data.firstBit = 0;
+    data.endBit = 32;
+    data.earlyExit = true;
+    CHECK(sortArray(engine, &data) == SORT_STATUS_SUCCESS);
+    CHECK(hipMemcpy(ok.data(), data.pKeys[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+    CHECK(hipMemcpy(o1.data(), data.pValues1[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return keys[a] < keys[b]; });
+    for (int i = 0; i < n; ++i) {
+        CHECK(ok[i] == keys[perm[i]]);
+        CHECK(o1[i] == (uint32_t)perm[i]);
+    }
+    data.Reset();
+    (void)hipFree(dk);
+    (void)hipFree(dv1);
+    (void)hipFree(dv2);
+    return 0;
+}
+
+// Keys of `words` 32-bit words (word 0 most significant), sorted LSD by word.
+static int bwt_style_test(MgpuSort& engine) {
+    const int n = 50000, words = 3;
+    std::mt19937 rng(11);
+    std::vector<std::vector<uint32_t>> host(words, std::vector<uint32_t>(n));
+    for (int w = 0; w < words; ++w)
+        for (int i = 0; i < n; ++i) host[w][i] = rng() & (w == 0 ? 0xFFu : 0xFFFFFFFFu);
+    // streams[w][p]: the two buffers of word w; streams[words] the indices.
+    uint32_t* streams[words + 1][2];
+    for (int w = 0; w <= words; ++w)
+        for (int p = 0; p < 2; ++p) CHECK((streams[w][p] = dalloc(n)) != nullptr);
+    for (int w = 0; w < words; ++w)
+        CHECK(hipMemcpy(streams[w][0], host[w].data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+    std::vector<uint32_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0u);
+    CHECK(hipMemcpy(streams[words][0], idx.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+
+    sortData_d d;
+    std::memset(&d, 0, sizeof(d));
+    d.maxElements = d.numElements = n;
+    d.parity = 0;
+    for (int cur = words - 1; cur >= 0; --cur) {
+        const int p = d.parity;
+        d.pKeys[0] = streams[cur][p];
+        d.pKeys[1] = streams[cur][1 - p];
+        for (int i = 0; i < cur; ++i) {
+            d.pValues[i][0] = streams[i][p];
+            d.pValues[i][1] = streams[i][1 - p];
+        }
+        d.pValues[cur][0] = streams[words][p];
+        d.pValues[cur][1] = streams[words][1 - p];
+        d.valueCount = cur + 1;
+        d.firstBit = 0;
+        d.endBit = cur == 0 ? 8 : 32;  // word 0 holds 8 significant bits
+        CHECK(sortArray(engine, &d) == SORT_STATUS_SUCCESS);
+    }
+    // After the chain, word 0 and the indices are at side parity.
+    std::vector<uint32_t> outIdx(n), outW0(n);
+    CHECK(hipMemcpy(outIdx.data(), streams[words][d.parity], 4 * n, hipMemcpyDeviceToHost) ==
+          hipSuccess);
+    CHECK(hipMemcpy(outW0.data(), streams[0][d.parity], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
+        for (int w = 0; w < words; ++w)
+            if (host[w][a] != host[w][b]) return host[w][a] < host[w][b];
+        return false;
+    });
+    for (int i = 0; i < n; ++i) {
+        CHECK(outIdx[i] == (uint32_t)perm[i]);
+        CHECK(outW0[i] == host[0][perm[i]]);
+    }
+    for (int w = 0; w <= words; ++w)
+        for (int p = 0; p < 2; ++p) (void)hipFree(streams[w][p]);
+    return 0;
+}
+
+int main(int argc, char** argv) {
+    const char* path = argc > 1 ? argv[1] : nullptr;
+    try {
+        MgpuSort engine(path);
+        if (simple_test(engine)) return 1;
+        if (bwt_style_test(engine)) return 1;
+    } catch (const MgpuException& e) {
+        std::printf("MgpuException: %s\n", e.what());
+        return 2;
+    }
+    std::printf("CLIENT OK\n");
+    return 0;
+}
