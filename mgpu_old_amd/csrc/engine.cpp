@@ -204,7 +204,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         ca.check_mask = totalBits >= 32 ? 0xffffffffu : ((1u << totalBits) - 1u);
     }
     uint32_t grid = (n + 8191) / 8192;
-    uint32_t maxGrid = (uint32_t)e->numCUs * 4;
+    uint32_t maxGrid = (uint32_t)e->numCUs * 8;
     if (grid > maxGrid) grid = maxGrid;
     if (grid < 1) grid = 1;
     ca.chunk = (uint32_t)(((uint64_t)n + grid - 1) / grid);

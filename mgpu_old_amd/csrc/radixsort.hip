@@ -119,18 +119,33 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
     auto bf = [&](uint32_t k) { return (k >> cs) & cm; };
     if (begin < end) {
         if (a.vec_ok) {
-            const uint4* v = reinterpret_cast<const uint4*>(a.keys + begin);
+            typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4* v = reinterpret_cast<const u32x4*>(a.keys + begin);
             const uint32_t nvec = (end - begin) >> 2;
-            for (uint32_t i = tid; i < nvec; i += BLOCK) {
-                uint4 q = v[i];
-                count_key(q.x);
-                count_key(q.y);
-                count_key(q.z);
-                count_key(q.w);
-                if (check) {
-                    descents += (bf(q.x) > bf(q.y)) + (bf(q.y) > bf(q.z)) + (bf(q.z) > bf(q.w));
-                    const uint32_t next = begin + 4 * i + 4;
-                    if (next < a.n) descents += bf(q.w) > bf(a.keys[next]);
+            // UNROLL independent 16-byte loads in flight per thread before any
+            // of them is used: the loop is bound by memory latency otherwise.
+            constexpr int UNROLL = 4;
+            for (uint32_t i0 = tid; i0 < nvec; i0 += UNROLL * BLOCK) {
+                u32x4 q[UNROLL];
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    const uint32_t i = i0 + u * BLOCK;
+                    if (i < nvec) q[u] = __builtin_nontemporal_load(v + i);
+                }
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    const uint32_t i = i0 + u * BLOCK;
+                    if (i >= nvec) break;
+                    count_key(q[u].x);
+                    count_key(q[u].y);
+                    count_key(q[u].z);
+                    count_key(q[u].w);
+                    if (check) {
+                        descents += (bf(q[u].x) > bf(q[u].y)) + (bf(q[u].y) > bf(q[u].z)) +
+                                    (bf(q[u].z) > bf(q[u].w));
+                        const uint32_t next = begin + 4 * i + 4;
+                        if (next < a.n) descents += bf(q[u].w) > bf(a.keys[next]);
+                    }
                 }
             }
             for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) {
