@@ -78,6 +78,32 @@ sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, i
     return ensure_lookback(lane, (uint64_t)passes * tiles * MGPU_RADIX);
 }
 
+sortStatus_t mgpu_ensure_lookback_words(SortLane& lane, uint64_t words) {
+    return ensure_lookback(lane, words);
+}
+
+sortStatus_t mgpu_ensure_multi(SortLane& L, uint32_t arrays) {
+    if (arrays <= L.multiArrays && !L.multiDirty) return SORT_STATUS_SUCCESS;
+    if (hipStreamSynchronize(L.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    if (arrays > L.multiArrays) {
+        if (L.multi) (void)hipFree(L.multi);
+        L.multi = nullptr;
+        L.multiArrays = 0;
+        const uint32_t cap = arrays + arrays / 2 + 16;
+        if (hipMalloc(&L.multi, (size_t)cap * 2 * MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t)) !=
+            hipSuccess) {
+            L.multi = nullptr;
+            return SORT_STATUS_DEVICE_ALLOC_FAILED;
+        }
+        L.multiArrays = cap;
+    }
+    if (hipMemset(L.multi, 0, (size_t)L.multiArrays * MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t)) !=
+        hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    L.multiDirty = false;
+    return SORT_STATUS_SUCCESS;
+}
+
 sortStatus_t mgpu_lane_init(SortLane* l) {
     const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_LANE_COUNTERS;
     if (hipMalloc(&l->fixed, words * sizeof(uint32_t)) != hipSuccess) {
@@ -94,6 +120,9 @@ sortStatus_t mgpu_lane_init(SortLane* l) {
 }
 
 void mgpu_lane_free(SortLane* l) {
+    if (l->multi) (void)hipFree(l->multi);
+    l->multi = nullptr;
+    l->multiArrays = 0;
     if (l->lookback) (void)hipFree(l->lookback);
     if (l->fixed) (void)hipFree(l->fixed);
     l->lookback = nullptr;

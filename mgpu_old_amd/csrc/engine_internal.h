@@ -32,6 +32,11 @@ struct SortLane {
     bool histDirty = false;  // hist/tileCounters may hold a partial sort's counts
     uint32_t* lookback = nullptr;  // grow-only
     uint64_t lookbackWords = 0;
+    // Device-wide passes over several arrays: hist and digitStart per array
+    // ([arrays][4][256] each, grow-only, hist zero between sorts).
+    uint32_t* multi = nullptr;
+    uint32_t multiArrays = 0;
+    bool multiDirty = false;
 };
 
 sortStatus_t mgpu_lane_init(SortLane* l);
@@ -49,7 +54,10 @@ struct sortEngine_d {
     hipFunction_t fnScan = nullptr;
     hipFunction_t fnIota = nullptr;
     hipFunction_t fnPass[4] = {};  // by value mode (MGPU_VALUES_*)
-    SortConfig config = {256, 16, 0};
+    hipFunction_t fnMultiCount = nullptr;
+    hipFunction_t fnMultiScan = nullptr;
+    hipFunction_t fnMultiPass[4] = {};
+    SortConfig config = {512, 16, 0};
 
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;
@@ -87,6 +95,9 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
 // Grows the lane's lookback words to cover a device-wide sort of n keys in
 // `passes` passes (a reallocation synchronizes the lane's stream).
 sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes);
+sortStatus_t mgpu_ensure_lookback_words(SortLane& lane, uint64_t words);
+// Grows the per-array scratch of the multi-array passes to `arrays` arrays.
+sortStatus_t mgpu_ensure_multi(SortLane& lane, uint32_t arrays);
 
 // Runs numPasses digit passes over n keys starting at firstBit.  Executed pass
 // k reads buffer side (k & 1) and writes side (k & 1) ^ 1 of keyBuf / valBuf,

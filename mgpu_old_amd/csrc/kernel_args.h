@@ -94,6 +94,52 @@ struct mgpuSegArgs {
     uint32_t mask[MGPU_SEG_MAX_PASSES];
 };
 
+// ---------------------------------------------------------------------------
+// Device-wide passes over several arrays at once (large segments): the tiles
+// of all arrays form one index space, each array's lookback chain starting at
+// its own first tile.
+
+// digit_count over chunks of the arrays: chunk c = keys [begin, end) of
+// array `seg`, read from side 0.
+struct mgpuMultiCountArgs {
+    const mgpuSegDesc* segs;
+    const uint32_t* chunks;  // [num_chunks][4]: seg, begin, end, unused
+    uint32_t* hist;          // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX], zero on entry
+    uint32_t* clear;         // lookback words to zero
+    uint64_t clear_words;    // multiple of 4
+    uint32_t num_chunks;
+    uint32_t num_passes;
+    uint32_t shift[MGPU_MAX_PASSES];
+    uint32_t mask[MGPU_MAX_PASSES];
+};
+
+// digit_scan, one workgroup per array; zeroes hist and the tile counters.
+struct mgpuMultiScanArgs {
+    uint32_t* hist;
+    uint32_t* digit_start;    // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX]
+    uint32_t* tile_counters;  // [MGPU_LANE_COUNTERS]
+    uint32_t num_passes;
+    uint32_t pad_;
+};
+
+// rank_scatter over the tiles of all arrays.
+struct mgpuMultiPassArgs {
+    const mgpuSegDesc* segs;
+    const uint32_t* tile_begin;   // [num_segs + 1]: first global tile of each array
+    const uint32_t* digit_start;  // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX]
+    uint32_t* lookback;           // [num_tiles][MGPU_RADIX] for this pass, zero on entry
+    uint32_t* tile_counter;       // zero on entry
+    uint32_t* error_flag;
+    uint32_t num_segs;
+    uint32_t num_tiles;
+    uint32_t pass;                // index into digit_start
+    uint32_t in_side;             // buffers read: side in_side, written: in_side ^ 1
+    uint32_t shift;
+    uint32_t mask;
+    uint32_t num_values;
+    uint32_t pad_;
+};
+
 // rank_scatter: one stable digit pass (rank in the tile, decoupled lookback
 // across tiles, LDS-staged scatter).
 struct mgpuPassArgs {

@@ -24,7 +24,7 @@ sortStatus_t get_fn(sortEngine_d* e, hipModule_t m, const std::string& name, hip
 // MGPUSORT_RANK=<0|1> pick one of the built variants (for sweeps); otherwise
 // the default.
 SortConfig pick_config() {
-    SortConfig c = {256, 16, 0};
+    SortConfig c = {512, 16, 0};
     const char* t = std::getenv("MGPUSORT_TILE");
     if (t && *t) {
         unsigned b = 0, i = 0;
@@ -50,6 +50,14 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
     e->config = pick_config();
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_scan", &e->fnMultiScan);
+    for (int v = 0; v < 4 && !st; ++v) {
+        char name[96];
+        std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_r%u_v%d",
+                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod, v);
+        st = get_fn(e, e->radixModule, name, &e->fnMultiPass[v]);
+    }
     for (int v = 0; v < 4 && !st; ++v) {
         char name[96];
         std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u_v%d",

@@ -189,9 +189,89 @@ __device__ void digit_scan_body(const mgpuScanArgs& a) {
 }
 
 // ---------------------------------------------------------------------------
+// Device-wide passes over several arrays: digit_count per chunk of an array
+// (one chunk per workgroup), digit_scan per array.
+template <int BLOCK>
+__device__ void multi_count_body(const mgpuMultiCountArgs& a) {
+    constexpr int W = BLOCK / 64;
+    __shared__ uint32_t s_hist[W][MGPU_MAX_PASSES][RADIX];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    uint32_t* flat = &s_hist[0][0][0];
+    for (uint32_t i = tid; i < (uint32_t)(W * MGPU_MAX_PASSES * RADIX); i += BLOCK) flat[i] = 0;
+    {
+        uint4* c = reinterpret_cast<uint4*>(a.clear);
+        const uint64_t nvec = a.clear_words >> 2;
+        for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + tid; i < nvec;
+             i += (uint64_t)gridDim.x * BLOCK)
+            c[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (blockIdx.x >= a.num_chunks) return;  // clearing only (uniform per workgroup)
+    __syncthreads();
+    const uint32_t* ch = a.chunks + 4 * blockIdx.x;
+    const uint32_t sg = ch[0], begin = ch[1], end = ch[2];
+    const uint32_t* keys = a.segs[sg].keys[0];
+    const uint32_t np = a.num_passes;
+    uint32_t (*h)[RADIX] = s_hist[wave];
+    auto count_key = [&](uint32_t k) {
+#pragma unroll
+        for (int p = 0; p < MGPU_MAX_PASSES; ++p)
+            if ((uint32_t)p < np)
+                __hip_atomic_fetch_add(&h[p][(k >> a.shift[p]) & a.mask[p]], 1u, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    // Scalar head up to 16-byte alignment, vector body, scalar tail.
+    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(keys + begin) >> 2) & 3);
+    const uint32_t vbeg = min(end, begin + ((4 - mis) & 3));
+    const uint32_t nvec = (end - vbeg) >> 2;
+    for (uint32_t i = begin + tid; i < vbeg; i += BLOCK) count_key(keys[i]);
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* v = reinterpret_cast<const u32x4*>(keys + vbeg);
+    constexpr int UNROLL = 4;
+    for (uint32_t i0 = tid; i0 < nvec; i0 += UNROLL * BLOCK) {
+        u32x4 q[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+            if (i0 + u * BLOCK < nvec) q[u] = __builtin_nontemporal_load(v + i0 + u * BLOCK);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            if (i0 + u * BLOCK >= nvec) break;
+            count_key(q[u].x);
+            count_key(q[u].y);
+            count_key(q[u].z);
+            count_key(q[u].w);
+        }
+    }
+    for (uint32_t i = vbeg + 4 * nvec + tid; i < end; i += BLOCK) count_key(keys[i]);
+    __syncthreads();
+    uint32_t* hist = a.hist + (size_t)sg * MGPU_MAX_PASSES * RADIX;
+    for (uint32_t i = tid; i < (uint32_t)(MGPU_MAX_PASSES * RADIX); i += BLOCK) {
+        if (i / RADIX >= np) break;
+        uint32_t s = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s += flat[w * MGPU_MAX_PASSES * RADIX + i];
+        if (s) __hip_atomic_fetch_add(&hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ void multi_scan_body(const mgpuMultiScanArgs& a) {
+    __shared__ uint32_t s_tmp[4];
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const size_t base = (size_t)blockIdx.x * MGPU_MAX_PASSES * RADIX;
+    for (uint32_t p = 0; p < a.num_passes; ++p) {
+        const uint32_t c = a.hist[base + p * RADIX + tid];
+        const uint32_t e = block_exclusive_scan<256>(c, s_tmp, lane, wave);
+        a.digit_start[base + p * RADIX + tid] = e;
+        a.hist[base + p * RADIX + tid] = 0;
+    }
+    if (blockIdx.x == 0 && tid < MGPU_LANE_COUNTERS) a.tile_counters[tid] = 0;
+}
+
+// ---------------------------------------------------------------------------
 // rank_scatter: one digit pass over one tile of BLOCK*IPT keys per workgroup.
-template <int BLOCK, int IPT, int RANK, int VMODE>
-__device__ void rank_scatter_body(const mgpuPassArgs& a) {
+// MULTI: the tile belongs to one of several arrays (mgpuMultiPassArgs);
+// otherwise to the single array of mgpuPassArgs.
+template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, typename Args>
+__device__ void rank_scatter_body(const Args& a) {
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
     constexpr uint32_t TILE = BLOCK * IPT;
@@ -203,26 +283,72 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
     __shared__ uint32_t s_global[RADIX];
     __shared__ uint32_t s_tmp[W];
     __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_seg;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
 
-    if (tid == 0)
-        s_tile = __hip_atomic_fetch_add(a.tile_counter, 1u, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) {
+        const uint32_t t = __hip_atomic_fetch_add(a.tile_counter, 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
+        s_tile = t;
+        if constexpr (MULTI) {
+            // The array holding global tile t: tile_begin[s] <= t < tile_begin[s+1].
+            uint32_t lo = 0, hi = a.num_segs;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.tile_begin[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            s_seg = lo;
+        }
+    }
     for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
         (&s_count[0][0])[i] = 0;
         if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
     }
     __syncthreads();
-    const uint32_t tile = s_tile;
-    if (tile >= a.num_tiles) {  // only a counter left dirty could do this
+    const uint32_t gtile = s_tile;
+    if (gtile >= a.num_tiles) {  // only a counter left dirty could do this
         if (tid == 0) st_agent(a.error_flag, 2u);
         return;
     }
 
-    const uint32_t n = a.n;
+    // The array this tile belongs to.
+    const uint32_t* keys_in;
+    uint32_t* keys_out;
+    const uint32_t* digit_start;
+    uint32_t* lookback;  // lookback words from the array's first tile on
+    uint32_t n, tile;
+    const mgpuSegDesc* sd = nullptr;
+    if constexpr (MULTI) {
+        const uint32_t sg = s_seg;
+        sd = &a.segs[sg];
+        const uint32_t tb = a.tile_begin[sg];
+        keys_in = sd->keys[a.in_side];
+        keys_out = sd->keys[a.in_side ^ 1];
+        n = sd->n;
+        tile = gtile - tb;
+        digit_start = a.digit_start + (sg * MGPU_MAX_PASSES + a.pass) * RADIX;
+        lookback = a.lookback + (size_t)tb * RADIX;
+    } else {
+        keys_in = a.keys_in;
+        keys_out = a.keys_out;
+        n = a.n;
+        tile = gtile;
+        digit_start = a.digit_start;
+        lookback = a.lookback;
+    }
+    auto vals_in = [&](uint32_t v) -> const uint32_t* {
+        if constexpr (MULTI) return sd->vals[v][a.in_side];
+        else return a.vals_in[v];
+    };
+    auto vals_out = [&](uint32_t v) -> uint32_t* {
+        if constexpr (MULTI) return sd->vals[v][a.in_side ^ 1];
+        else return a.vals_out[v];
+    };
+
     const uint32_t base = tile * TILE;
     const uint32_t wave_base = base + wave * WAVE_TILE;
     const uint32_t tile_valid = min(TILE, n - base);
@@ -234,22 +360,22 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
     uint32_t val[IPT];
     if (full) {
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) key[j] = a.keys_in[wave_base + j * 64 + lane];
+        for (int j = 0; j < IPT; ++j) key[j] = keys_in[wave_base + j * 64 + lane];
         if constexpr (VMODE == MGPU_VALUES_SINGLE) {
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) val[j] = a.vals_in[0][wave_base + j * 64 + lane];
+            for (int j = 0; j < IPT; ++j) val[j] = vals_in(0)[wave_base + j * 64 + lane];
         }
     } else {
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             uint32_t idx = wave_base + j * 64 + lane;
-            key[j] = idx < n ? a.keys_in[idx] : 0xffffffffu;
+            key[j] = idx < n ? keys_in[idx] : 0xffffffffu;
         }
         if constexpr (VMODE == MGPU_VALUES_SINGLE) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 uint32_t idx = wave_base + j * 64 + lane;
-                val[j] = idx < n ? a.vals_in[0][idx] : 0u;
+                val[j] = idx < n ? vals_in(0)[idx] : 0u;
             }
         }
     }
@@ -307,7 +433,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
         }
         total = sum;
         tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
-        st_agent(&a.lookback[tile * RADIX + tid],
+        st_agent(&lookback[tile * RADIX + tid],
                  (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
     }
     const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
@@ -317,7 +443,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
             int32_t t = (int32_t)tile - 1;
             uint32_t spins = 0;
             while (true) {
-                uint32_t s = ld_acquire_agent(&a.lookback[(uint32_t)t * RADIX + tid]);
+                uint32_t s = ld_acquire_agent(&lookback[(uint32_t)t * RADIX + tid]);
                 if (!(s & MGPU_LB_STATE)) {
                     if (++spins > kSpinLimit) {
                         st_agent(a.error_flag, 1u);
@@ -334,9 +460,9 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
                 }
                 --t;
             }
-            st_agent(&a.lookback[tile * RADIX + tid], MGPU_LB_PREFIX | (prefix + tile_count));
+            st_agent(&lookback[tile * RADIX + tid], MGPU_LB_PREFIX | (prefix + tile_count));
         }
-        s_global[tid] = a.digit_start[tid] + prefix - local_start;
+        s_global[tid] = digit_start[tid] + prefix - local_start;
 #pragma unroll
         for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
     }
@@ -363,7 +489,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
             dest[i] = s_global[(k >> shift) & mask] + p;
             // Offsets outside [0, n) can only come from a broken count or
             // lookback: flag it instead of writing out of bounds.
-            if (dest[i] < n) a.keys_out[dest[i]] = k;
+            if (dest[i] < n) keys_out[dest[i]] = k;
             else st_agent(a.error_flag, 4u);
         }
     }
@@ -375,7 +501,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) val[j] = wave_base + j * 64 + lane;
             } else if constexpr (VMODE == MGPU_VALUES_MULTI) {
-                const uint32_t* src = a.vals_in[v];
+                const uint32_t* src = vals_in(v);
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     uint32_t idx = wave_base + j * 64 + lane;
@@ -386,7 +512,7 @@ __device__ void rank_scatter_body(const mgpuPassArgs& a) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
             __syncthreads();
-            uint32_t* dst = a.vals_out[v];
+            uint32_t* dst = vals_out(v);
 #pragma unroll
             for (int i = 0; i < IPT; ++i) {
                 const uint32_t p = i * BLOCK + tid;
@@ -413,10 +539,22 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_iota(mgpuIotaArgs a) {
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += gridDim.x * 256) a.out[i] = i;
 }
 
+extern "C" __global__ void __launch_bounds__(256) mgpu_multi_count(mgpuMultiCountArgs a) {
+    multi_count_body<256>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanArgs a) {
+    multi_scan_body(a);
+}
+
 #define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
-        rank_scatter_body<B, I, R, MODE>(a);                                                  \
+        rank_scatter_body<B, I, R, MODE, false>(a);                                           \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
+        rank_scatter_body<B, I, R, MODE, true>(a);                                            \
     }
 #define MGPU_RANK_SCATTER(B, I, R)                                                            \
     MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
@@ -425,8 +563,9 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_iota(mgpuIotaArgs a) {
     MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
 
 MGPU_RANK_SCATTER(256, 16, 0)
-MGPU_RANK_SCATTER(256, 16, 1)
 MGPU_RANK_SCATTER(256, 24, 0)
-MGPU_RANK_SCATTER(256, 24, 1)
 MGPU_RANK_SCATTER(512, 16, 0)
 MGPU_RANK_SCATTER(512, 16, 1)
+MGPU_RANK_SCATTER(512, 24, 0)
+MGPU_RANK_SCATTER(512, 32, 0)
+MGPU_RANK_SCATTER(1024, 16, 0)

@@ -20,9 +20,11 @@
 
 namespace {
 
-// Segments above this size go to the device-wide path (one rank_scatter
-// launch per pass) instead of one workgroup.
-constexpr uint32_t kDeviceWideMin = 1u << 18;
+// Segments of at least this size go to the device-wide passes (all of them
+// together) instead of one workgroup each.
+constexpr uint32_t kDeviceWideMin = (1u << 16) + 1;
+// Keys per digit_count workgroup on that path.
+constexpr uint32_t kWideChunk = 1u << 16;
 
 const SegTuneEntry kDefaultTable[] = {
     // max log2(n), radix bits, items per thread
@@ -176,10 +178,35 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         P.launches.push_back(L);
     }
 
-    const size_t descBytes = sizeof(mgpuSegDesc) * (size_t)std::max(1, numSegments);
-    const size_t listBytes = (sizeof(uint32_t) * std::max<size_t>(1, lists.size()) + 255) & ~255ull;
+    // Device-wide segments: their descriptors, first tile of each, and the
+    // digit_count chunks.
+    std::vector<mgpuSegDesc> wideDescs;
+    std::vector<uint32_t> tileBegin, chunks;
+    if (!P.wide.empty()) {
+        const uint32_t tile = e->config.blockThreads * e->config.itemsPerThread;
+        tileBegin.push_back(0);
+        for (size_t k = 0; k < P.wide.size(); ++k) {
+            const mgpuSegDesc& d = segs[P.wide[k]];
+            wideDescs.push_back(d);
+            tileBegin.push_back(tileBegin.back() + (d.n + tile - 1) / tile);
+            for (uint32_t b = 0; b < d.n; b += kWideChunk) {
+                chunks.push_back((uint32_t)k);
+                chunks.push_back(b);
+                chunks.push_back(std::min(d.n, b + kWideChunk));
+                chunks.push_back(0);
+            }
+        }
+        P.wideTiles = tileBegin.back();
+        P.wideChunks = (uint32_t)(chunks.size() / 4);
+    }
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t descBytes = al(sizeof(mgpuSegDesc) * (size_t)std::max(1, numSegments));
+    const size_t listBytes = al(sizeof(uint32_t) * std::max<size_t>(1, lists.size()));
+    const size_t wideBytes = al(sizeof(mgpuSegDesc) * wideDescs.size());
+    const size_t tbBytes = al(sizeof(uint32_t) * tileBegin.size());
+    const size_t chBytes = al(sizeof(uint32_t) * chunks.size());
     const size_t ctrWords = 4 * std::max<size_t>(1, P.launches.size());
-    const size_t payload = descBytes + listBytes;
+    const size_t payload = descBytes + listBytes + wideBytes + tbBytes + chBytes;
     const size_t total = payload + ctrWords * sizeof(uint32_t);
     char* base = nullptr;
     if (!arena) {
@@ -194,6 +221,15 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         if (!lists.empty() && hipMemcpy(base + descBytes, lists.data(),
                                         sizeof(uint32_t) * lists.size(),
                                         hipMemcpyHostToDevice) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        size_t off = descBytes + listBytes;
+        if (!wideDescs.empty() &&
+            (hipMemcpy(base + off, wideDescs.data(), sizeof(mgpuSegDesc) * wideDescs.size(),
+                       hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(base + off + wideBytes, tileBegin.data(), sizeof(uint32_t) * tileBegin.size(),
+                       hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(base + off + wideBytes + tbBytes, chunks.data(),
+                       sizeof(uint32_t) * chunks.size(), hipMemcpyHostToDevice) != hipSuccess))
             return SORT_STATUS_DEVICE_ERROR;
         if (hipMemset(base + payload, 0, ctrWords * sizeof(uint32_t)) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
@@ -225,6 +261,12 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         char* h = static_cast<char*>(arena->host);
         if (numSegments) std::memcpy(h, segs, sizeof(mgpuSegDesc) * numSegments);
         if (!lists.empty()) std::memcpy(h + descBytes, lists.data(), sizeof(uint32_t) * lists.size());
+        if (!wideDescs.empty()) {
+            const size_t off = descBytes + listBytes;
+            std::memcpy(h + off, wideDescs.data(), sizeof(mgpuSegDesc) * wideDescs.size());
+            std::memcpy(h + off + wideBytes, tileBegin.data(), sizeof(uint32_t) * tileBegin.size());
+            std::memcpy(h + off + wideBytes + tbBytes, chunks.data(), sizeof(uint32_t) * chunks.size());
+        }
         if (hipMemcpyAsync(base, h, payload, hipMemcpyHostToDevice, lane.stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
         P.devCounters = reinterpret_cast<uint32_t*>(base + arena->hostBytes);
@@ -232,18 +274,99 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     P.devSegs = reinterpret_cast<mgpuSegDesc*>(base);
     P.devLists = reinterpret_cast<uint32_t*>(base + descBytes);
     if (!arena) P.devCounters = reinterpret_cast<uint32_t*>(base + payload);
+    P.devWide = reinterpret_cast<mgpuSegDesc*>(base + descBytes + listBytes);
+    P.devTileBegin = reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes);
+    P.devChunks = reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes + tbBytes);
 
-    // Device-wide segments need lookback words before any capture.
-    uint32_t maxWide = 0;
-    for (uint32_t i : P.wide) maxWide = std::max(maxWide, segs[i].n);
-    if (maxWide) {
-        sortStatus_t st = mgpu_ensure_lookback(e, lane, maxWide, P.sortPasses);
+    // Device-wide segments: lookback words and per-array scratch before any
+    // capture.
+    if (!P.wide.empty()) {
+        sortStatus_t st = mgpu_ensure_lookback_words(
+            lane, (uint64_t)P.sortPasses * P.wideTiles * MGPU_RADIX);
+        if (!st) st = mgpu_ensure_multi(lane, (uint32_t)P.wide.size());
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;
 }
 
-// Queues the launches of a built plan on the engine's stream.
+namespace {
+
+// The device-wide passes over every large segment of the plan at once:
+// digit_count over their chunks, digit_scan per segment, then one
+// rank_scatter launch per pass over the tiles of all of them.
+sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
+    sortEngine_d* e = P.engine;
+    const uint32_t np = (uint32_t)P.sortPasses;
+    const uint64_t lbWords = (uint64_t)np * P.wideTiles * MGPU_RADIX;
+    const uint32_t numWide = (uint32_t)P.wide.size();
+    uint32_t* hist = lane.multi;
+    uint32_t* digitStart = lane.multi + (size_t)lane.multiArrays * MGPU_MAX_PASSES * MGPU_RADIX;
+
+    mgpuMultiCountArgs ca;
+    std::memset(&ca, 0, sizeof(ca));
+    ca.segs = P.devWide;
+    ca.chunks = P.devChunks;
+    ca.hist = hist;
+    ca.clear = lane.lookback;
+    ca.clear_words = (lbWords + 3) & ~uint64_t(3);
+    ca.num_chunks = P.wideChunks;
+    ca.num_passes = np;
+    int bit = P.firstBit;
+    for (uint32_t p = 0; p < np; ++p) {
+        ca.shift[p] = (uint32_t)bit;
+        ca.mask[p] = (1u << P.sortWidths[p]) - 1u;
+        bit += P.sortWidths[p];
+    }
+    uint32_t grid = P.wideChunks;
+    const uint32_t clearGrid = (uint32_t)std::min<uint64_t>(
+        (ca.clear_words / 4 + 256 * 16 - 1) / (256 * 16), (uint64_t)e->numCUs * 8);
+    grid = std::max(grid, clearGrid);
+    lane.multiDirty = true;
+    sortStatus_t st = mgpu_launch(e, lane.stream, e->fnMultiCount, grid, 256, &ca);
+    if (st) return st;
+
+    mgpuMultiScanArgs sa;
+    sa.hist = hist;
+    sa.digit_start = digitStart;
+    sa.tile_counters = lane.tileCounters;
+    sa.num_passes = np;
+    sa.pad_ = 0;
+    st = mgpu_launch(e, lane.stream, e->fnMultiScan, numWide, 256, &sa);
+    if (st) return st;
+    lane.multiDirty = false;
+
+    const int carried = P.valueCount == -1 ? 1 : P.valueCount;
+    for (uint32_t p = 0; p < np; ++p) {
+        mgpuMultiPassArgs pa;
+        std::memset(&pa, 0, sizeof(pa));
+        pa.segs = P.devWide;
+        pa.tile_begin = P.devTileBegin;
+        pa.digit_start = digitStart;
+        pa.lookback = lane.lookback + (uint64_t)p * P.wideTiles * MGPU_RADIX;
+        pa.tile_counter = lane.tileCounters + p;
+        pa.error_flag = e->errorFlag;
+        pa.num_segs = numWide;
+        pa.num_tiles = P.wideTiles;
+        pa.pass = p;
+        pa.in_side = p & 1;
+        pa.shift = ca.shift[p];
+        pa.mask = ca.mask[p];
+        pa.num_values = (uint32_t)carried;
+        int mode;
+        if (P.valueCount == 0) mode = MGPU_VALUES_NONE;
+        else if (P.valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
+        else if (P.valueCount == 1) mode = MGPU_VALUES_SINGLE;
+        else mode = MGPU_VALUES_MULTI;
+        st = mgpu_launch(e, lane.stream, e->fnMultiPass[mode], P.wideTiles,
+                         e->config.blockThreads, &pa);
+        if (st) return st;
+    }
+    return SORT_STATUS_SUCCESS;
+}
+
+}  // namespace
+
+// Queues the launches of a built plan on the lane's stream.
 sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
     sortEngine_d* e = P.engine;
     for (size_t li = 0; li < P.launches.size(); ++li) {
@@ -270,18 +393,7 @@ sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
         sortStatus_t st = mgpu_launch(e, lane.stream, L.fn, L.grid, 256, &a);
         if (st) return st;
     }
-    for (uint32_t i : P.wide) {
-        const mgpuSegDesc& s = P.hostSegs[i];
-        uint32_t* keyBuf[2] = {s.keys[0], s.keys[1]};
-        uint32_t* valBuf[MGPU_MAX_VALUES][2];
-        for (int v = 0; v < MGPU_MAX_VALUES; ++v) {
-            valBuf[v][0] = s.vals[v][0];
-            valBuf[v][1] = s.vals[v][1];
-        }
-        sortStatus_t st = mgpu_sort_passes(e, lane, s.n, P.firstBit, P.sortPasses, P.sortWidths,
-                                           P.valueCount, keyBuf, valBuf);
-        if (st) return st;
-    }
+    if (!P.wide.empty()) return mgpu_enqueue_wide(P, lane);
     return SORT_STATUS_SUCCESS;
 }
 

@@ -63,6 +63,11 @@ struct SegPlanData {
     std::vector<mgpuSegDesc> hostSegs;
     std::vector<SegLaunch> launches;
     std::vector<uint32_t> wide;  // segments sorted by the device-wide path
+    uint32_t wideTiles = 0;      // tiles of all device-wide segments
+    uint32_t wideChunks = 0;     // digit_count chunks of them
+    mgpuSegDesc* devWide = nullptr;
+    uint32_t* devTileBegin = nullptr;
+    uint32_t* devChunks = nullptr;
     void* devBlock = nullptr;  // owned allocation (null when an arena holds the arrays)
     mgpuSegDesc* devSegs = nullptr;
     uint32_t* devLists = nullptr;

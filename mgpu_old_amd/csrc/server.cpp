@@ -34,6 +34,8 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr size_t kAlign = 256;
+// Smallest batch worth a slot of its own when admitted work is split.
+constexpr uint64_t kMinBatchKeys = 1ull << 22;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
@@ -272,20 +274,34 @@ void worker_main(sortServer_d* s) {
             s->waiting.pop_front();
             progress = true;
         }
-        // Launch everything admitted on a free slot.
+        // Launch what is admitted, spread over the free slots so that large
+        // requests on different slots run side by side: each free slot takes
+        // requests in arrival order up to an even share of the admitted keys
+        // (at least kMinBatchKeys, and always at least one request).
         if (!s->admitted.empty()) {
+            size_t freeSlots = 0;
+            for (Slot& slot : s->slots) freeSlots += !slot.busy;
+            uint64_t keys = 0;
+            for (const Pending& p : s->admitted) keys += (uint64_t)p.req.numElements;
+            const uint64_t share =
+                std::max<uint64_t>(kMinBatchKeys, freeSlots ? (keys + freeSlots - 1) / freeSlots : keys);
+            size_t next = 0;
             for (Slot& slot : s->slots) {
-                if (slot.busy) continue;
+                if (slot.busy || next >= s->admitted.size()) continue;
                 std::vector<Pending> batch;
-                batch.swap(s->admitted);
+                uint64_t taken = 0;
+                while (next < s->admitted.size() && (batch.empty() || taken < share)) {
+                    taken += (uint64_t)s->admitted[next].req.numElements;
+                    batch.push_back(s->admitted[next++]);
+                }
                 sortStatus_t st = launch_batch(s, slot, batch);
                 if (st) {
                     (void)hipStreamSynchronize(slot.lane.stream);
                     finish_slot(s, slot, st);
                 }
                 progress = true;
-                break;
             }
+            s->admitted.erase(s->admitted.begin(), s->admitted.begin() + (long)next);
         }
         if (!progress) {
             std::unique_lock<std::mutex> g(s->mu);
