@@ -310,7 +310,8 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         pa.shift = (uint32_t)shiftBits;
         pa.mask = (1u << widths[p]) - 1u;
         pa.num_values = (uint32_t)carried;
-        st = mgpu_launch(e, L.stream, e->fnPass[mode], numTiles, cfg.blockThreads, &pa);
+        const uint32_t grid = cfg.persistent ? std::min(numTiles, e->passGrid[mode]) : numTiles;
+        st = mgpu_launch(e, L.stream, e->fnPass[mode], grid, cfg.blockThreads, &pa);
         if (st) return st;
         ++done;
         if (executed) *executed = done;

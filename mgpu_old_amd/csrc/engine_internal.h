@@ -18,6 +18,7 @@ struct SortConfig {
     uint32_t blockThreads;
     uint32_t itemsPerThread;
     uint32_t rankMethod;
+    uint32_t persistent;  // workgroups loop over tiles (grid = resident workgroups)
 };
 
 // One in-order queue of sorts: a stream plus the device scratch of the
@@ -57,7 +58,9 @@ struct sortEngine_d {
     hipFunction_t fnMultiCount = nullptr;
     hipFunction_t fnMultiScan = nullptr;
     hipFunction_t fnMultiPass[4] = {};
-    SortConfig config = {512, 16, 0};
+    uint32_t passGrid[4] = {};       // persistent: resident workgroups of fnPass[mode]
+    uint32_t multiPassGrid[4] = {};  // same for fnMultiPass[mode]
+    SortConfig config = {512, 16, 0, 0};
 
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;

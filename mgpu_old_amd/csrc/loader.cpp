@@ -24,7 +24,7 @@ sortStatus_t get_fn(sortEngine_d* e, hipModule_t m, const std::string& name, hip
 // MGPUSORT_RANK=<0|1> pick one of the built variants (for sweeps); otherwise
 // the default.
 SortConfig pick_config() {
-    SortConfig c = {512, 16, 0};
+    SortConfig c = {512, 16, 0, 0};
     const char* t = std::getenv("MGPUSORT_TILE");
     if (t && *t) {
         unsigned b = 0, i = 0;
@@ -35,7 +35,19 @@ SortConfig pick_config() {
     }
     const char* r = std::getenv("MGPUSORT_RANK");
     if (r && (*r == '0' || *r == '1')) c.rankMethod = (uint32_t)(*r - '0');
+    const char* ps = std::getenv("MGPUSORT_PERSIST");
+    if (ps && (*ps == '0' || *ps == '1')) c.persistent = (uint32_t)(*ps - '0');
     return c;
+}
+
+// Workgroups of f that fit on the device at once (grid of a persistent launch).
+uint32_t resident_grid(const sortEngine_d* e, hipFunction_t f) {
+    int blocks = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, (int)e->config.blockThreads,
+                                                           0) != hipSuccess ||
+        blocks < 1)
+        blocks = 1;
+    return (uint32_t)blocks * (uint32_t)e->numCUs;
 }
 
 }  // namespace
@@ -52,17 +64,22 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     e->config = pick_config();
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_scan", &e->fnMultiScan);
+    const char* pers = e->config.persistent ? "_p" : "";
     for (int v = 0; v < 4 && !st; ++v) {
         char name[96];
-        std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_r%u_v%d",
-                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod, v);
+        std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_r%u%s_v%d",
+                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod,
+                      pers, v);
         st = get_fn(e, e->radixModule, name, &e->fnMultiPass[v]);
+        if (!st) e->multiPassGrid[v] = resident_grid(e, e->fnMultiPass[v]);
     }
     for (int v = 0; v < 4 && !st; ++v) {
         char name[96];
-        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u_v%d",
-                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod, v);
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u%s_v%d",
+                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod,
+                      pers, v);
         st = get_fn(e, e->radixModule, name, &e->fnPass[v]);
+        if (!st) e->passGrid[v] = resident_grid(e, e->fnPass[v]);
     }
     if (st) return st;
     st = mgpu_seg_load(e);

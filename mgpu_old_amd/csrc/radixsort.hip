@@ -267,10 +267,101 @@ __device__ void multi_scan_body(const mgpuMultiScanArgs& a) {
 }
 
 // ---------------------------------------------------------------------------
-// rank_scatter: one digit pass over one tile of BLOCK*IPT keys per workgroup.
-// MULTI: the tile belongs to one of several arrays (mgpuMultiPassArgs);
-// otherwise to the single array of mgpuPassArgs.
-template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, typename Args>
+// rank_scatter: one digit pass, one tile of BLOCK*IPT keys at a time.
+// MULTI: tiles of several arrays (mgpuMultiPassArgs), else of the single array
+// of mgpuPassArgs.  PERSIST: a workgroup loops over tiles in the order it
+// takes them from the tile counter, loading the next tile's keys while the
+// current one is looked back and scattered; otherwise one tile per workgroup.
+
+// Where one tile reads and writes.
+struct TileJob {
+    const uint32_t* keys_in;
+    uint32_t* keys_out;
+    const mgpuSegDesc* sd;  // MULTI: the array's descriptor
+    const uint32_t* digit_start;
+    uint32_t* lookback;     // lookback words from the array's first tile on
+    uint32_t n;             // keys of the array
+    uint32_t tile;          // tile index within the array
+};
+
+// Global tile t of a pass: which array (MULTI) and where its data lives.
+template <bool MULTI, typename Args>
+__device__ __forceinline__ uint32_t find_array(const Args& a, uint32_t t) {
+    if constexpr (MULTI) {
+        uint32_t lo = 0, hi = a.num_segs;  // tile_begin[lo] <= t < tile_begin[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.tile_begin[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    } else {
+        (void)a;
+        (void)t;
+        return 0;
+    }
+}
+
+template <bool MULTI, typename Args>
+__device__ __forceinline__ TileJob make_job(const Args& a, uint32_t gtile, uint32_t seg) {
+    TileJob j;
+    if constexpr (MULTI) {
+        j.sd = &a.segs[seg];
+        const uint32_t tb = a.tile_begin[seg];
+        j.keys_in = j.sd->keys[a.in_side];
+        j.keys_out = j.sd->keys[a.in_side ^ 1];
+        j.n = j.sd->n;
+        j.tile = gtile - tb;
+        j.digit_start = a.digit_start + (seg * MGPU_MAX_PASSES + a.pass) * RADIX;
+        j.lookback = a.lookback + (size_t)tb * RADIX;
+    } else {
+        (void)seg;
+        j.sd = nullptr;
+        j.keys_in = a.keys_in;
+        j.keys_out = a.keys_out;
+        j.n = a.n;
+        j.tile = gtile;
+        j.digit_start = a.digit_start;
+        j.lookback = a.lookback;
+    }
+    return j;
+}
+
+template <bool MULTI, typename Args>
+__device__ __forceinline__ const uint32_t* job_vals_in(const Args& a, const TileJob& j,
+                                                       uint32_t v) {
+    if constexpr (MULTI) return j.sd->vals[v][a.in_side];
+    else return a.vals_in[v];
+}
+
+template <bool MULTI, typename Args>
+__device__ __forceinline__ uint32_t* job_vals_out(const Args& a, const TileJob& j, uint32_t v) {
+    if constexpr (MULTI) return j.sd->vals[v][a.in_side ^ 1];
+    else return a.vals_out[v];
+}
+
+// Striped load of one array of a tile: slot j of lane l of wave w holds
+// element tile_base + w*64*IPT + j*64 + l; elements past n read as `fill`.
+// Addresses are one per-lane base plus constant offsets (j * 256 bytes), and
+// the bounds test compares constants against one per-lane remainder, so that
+// nothing per slot stays live across a tile loop.
+template <int IPT>
+__device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t wave_base,
+                                             uint32_t lane, uint32_t n, bool full, uint32_t fill,
+                                             uint32_t* out) {
+    const uint32_t off = wave_base + lane;
+    const uint32_t* p = src + off;
+    if (full) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) out[j] = __builtin_nontemporal_load(p + j * 64);
+    } else {
+        const int32_t rem = (int32_t)(n - off);
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) out[j] = j * 64 < rem ? p[j * 64] : fill;
+    }
+}
+
+template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool PERSIST, typename Args>
 __device__ void rank_scatter_body(const Args& a) {
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
@@ -282,242 +373,218 @@ __device__ void rank_scatter_body(const Args& a) {
     __shared__ uint64_t s_match[RANK == 0 ? W : 1][RANK == 0 ? RADIX : 1];
     __shared__ uint32_t s_global[RADIX];
     __shared__ uint32_t s_tmp[W];
-    __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_seg;
+    __shared__ uint32_t s_tile[2];
+    __shared__ uint32_t s_seg[2];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
+    const uint32_t shift = a.shift, mask = a.mask;
 
-    if (tid == 0) {
+    auto acquire = [&](int slot) {  // thread 0 only
         const uint32_t t = __hip_atomic_fetch_add(a.tile_counter, 1u, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
-        s_tile = t;
-        if constexpr (MULTI) {
-            // The array holding global tile t: tile_begin[s] <= t < tile_begin[s+1].
-            uint32_t lo = 0, hi = a.num_segs;
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (a.tile_begin[mid] <= t) lo = mid;
-                else hi = mid;
-            }
-            s_seg = lo;
-        }
-    }
+        s_tile[slot] = t;
+        s_seg[slot] = t < a.num_tiles ? find_array<MULTI>(a, t) : 0u;
+    };
+
+    if (tid == 0) acquire(0);
     for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
         (&s_count[0][0])[i] = 0;
         if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
     }
     __syncthreads();
-    const uint32_t gtile = s_tile;
-    if (gtile >= a.num_tiles) {  // only a counter left dirty could do this
-        if (tid == 0) st_agent(a.error_flag, 2u);
+    int cur = 0;
+    // Tile and array indices are the same for every lane: keep them (and all
+    // that is derived from them) in scalar registers.
+    uint32_t gtile = __builtin_amdgcn_readfirstlane(s_tile[0]);
+    if (gtile >= a.num_tiles) {
+        // One workgroup per tile: only a counter left dirty gets here.  A
+        // persistent workgroup just finds the pass already done.
+        if (!PERSIST && tid == 0) st_agent(a.error_flag, 2u);
         return;
     }
-
-    // The array this tile belongs to.
-    const uint32_t* keys_in;
-    uint32_t* keys_out;
-    const uint32_t* digit_start;
-    uint32_t* lookback;  // lookback words from the array's first tile on
-    uint32_t n, tile;
-    const mgpuSegDesc* sd = nullptr;
-    if constexpr (MULTI) {
-        const uint32_t sg = s_seg;
-        sd = &a.segs[sg];
-        const uint32_t tb = a.tile_begin[sg];
-        keys_in = sd->keys[a.in_side];
-        keys_out = sd->keys[a.in_side ^ 1];
-        n = sd->n;
-        tile = gtile - tb;
-        digit_start = a.digit_start + (sg * MGPU_MAX_PASSES + a.pass) * RADIX;
-        lookback = a.lookback + (size_t)tb * RADIX;
-    } else {
-        keys_in = a.keys_in;
-        keys_out = a.keys_out;
-        n = a.n;
-        tile = gtile;
-        digit_start = a.digit_start;
-        lookback = a.lookback;
-    }
-    auto vals_in = [&](uint32_t v) -> const uint32_t* {
-        if constexpr (MULTI) return sd->vals[v][a.in_side];
-        else return a.vals_in[v];
-    };
-    auto vals_out = [&](uint32_t v) -> uint32_t* {
-        if constexpr (MULTI) return sd->vals[v][a.in_side ^ 1];
-        else return a.vals_out[v];
-    };
-
-    const uint32_t base = tile * TILE;
-    const uint32_t wave_base = base + wave * WAVE_TILE;
-    const uint32_t tile_valid = min(TILE, n - base);
-    const bool full = tile_valid == TILE;
-    const uint32_t shift = a.shift, mask = a.mask;
-
-    // Load keys (and the single value array) striped: coalesced 256-byte rows.
+    TileJob job = make_job<MULTI>(a, gtile, __builtin_amdgcn_readfirstlane(s_seg[0]));
     uint32_t key[IPT];
-    uint32_t val[IPT];
-    if (full) {
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) key[j] = keys_in[wave_base + j * 64 + lane];
-        if constexpr (VMODE == MGPU_VALUES_SINGLE) {
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) val[j] = vals_in(0)[wave_base + j * 64 + lane];
-        }
-    } else {
+    {
+        const uint32_t base = job.tile * TILE;
+        load_striped<IPT>(job.keys_in, base + wave * WAVE_TILE, lane, job.n,
+                          job.n - base >= TILE, 0xffffffffu, key);
+    }
+
+    while (true) {
+        if (PERSIST && tid == 0) acquire(cur ^ 1);  // seen after the next barrier
+
+        const uint32_t n = job.n, tile = job.tile;
+        const uint32_t base = tile * TILE;
+        const uint32_t wave_base = base + wave * WAVE_TILE;
+        const uint32_t tile_valid = min(TILE, n - base);
+        const bool full = tile_valid == TILE;
+
+        uint32_t val[IPT];
+        if constexpr (VMODE == MGPU_VALUES_SINGLE)
+            load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), wave_base, lane, n, full, 0u, val);
+
+        // Digit of slot j; keys past n get the top digit so that they rank
+        // after every real key of the tile (they are last in input order).
+        const int32_t rem = (int32_t)(n - (wave_base + lane));  // slots j*64 < rem are real
+        auto digit_of = [&](int j) -> uint32_t {
+            if (full) return (key[j] >> shift) & mask;
+            return j * 64 < rem ? ((key[j] >> shift) & mask) : (RADIX - 1);
+        };
+
+        // Rank within the wave: rank[j] = earlier same-digit keys of this wave.
+        uint32_t pos[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
-            uint32_t idx = wave_base + j * 64 + lane;
-            key[j] = idx < n ? keys_in[idx] : 0xffffffffu;
-        }
-        if constexpr (VMODE == MGPU_VALUES_SINGLE) {
+            const uint32_t d = digit_of(j);
+            uint64_t m;
+            if constexpr (RANK == 1) {
+                m = ~0ull;
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                uint32_t idx = wave_base + j * 64 + lane;
-                val[j] = idx < n ? vals_in(0)[idx] : 0u;
+                for (int b = 0; b < MGPU_RADIX_BITS; ++b) {
+                    const uint32_t bit = (d >> b) & 1u;
+                    const uint64_t bb = __builtin_amdgcn_ballot_w64(bit != 0);
+                    m &= bit ? bb : ~bb;
+                }
+            } else {
+                __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            const uint32_t below = lanes_below(m);
+            const uint32_t old = s_count[wave][d];
+            pos[j] = old + below;
+            if (below == 0) {  // lowest lane of the group advances the count
+                s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
+                if constexpr (RANK == 0)
+                    __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
-    }
+        __syncthreads();
 
-    // Digit of slot j; keys past n get the top digit so that they rank after
-    // every real key of the tile (they are last in input order).
-    auto digit_of = [&](int j) -> uint32_t {
-        if (full) return (key[j] >> shift) & mask;
-        uint32_t idx = wave_base + j * 64 + lane;
-        return idx < n ? ((key[j] >> shift) & mask) : (RADIX - 1);
-    };
-
-    // Rank within the wave: rank[j] = earlier same-digit keys of this wave.
-    uint32_t rank[IPT];
+        // Per digit (thread d): wave-exclusive offsets and the tile total,
+        // published at once for the tiles after this one.
+        uint32_t total = 0, tile_count = 0;
+        if (tid < RADIX) {
+            uint32_t sum = 0;
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const uint32_t d = digit_of(j);
-        uint64_t m;
-        if constexpr (RANK == 1) {
-            m = ~0ull;
-#pragma unroll
-            for (int b = 0; b < MGPU_RADIX_BITS; ++b) {
-                const uint32_t bit = (d >> b) & 1u;
-                const uint64_t bb = __builtin_amdgcn_ballot_w64(bit != 0);
-                m &= bit ? bb : ~bb;
+            for (int w = 0; w < W; ++w) {
+                const uint32_t c = s_count[w][tid];
+                s_count[w][tid] = sum;
+                sum += c;
             }
-        } else {
-            __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-            m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            total = sum;
+            tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
+            st_agent(&job.lookback[tile * RADIX + tid],
+                     (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
         }
-        const uint32_t below = lanes_below(m);
-        const uint32_t old = s_count[wave][d];
-        rank[j] = old + below;
-        if (below == 0) {  // lowest lane of the group advances the count
-            s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
-            if constexpr (RANK == 0)
-                __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-        }
-    }
-    __syncthreads();
-
-    // Per digit (thread d): wave-exclusive offsets, tile total, publish the
-    // tile total for the lookback.
-    uint32_t total = 0, tile_count = 0;
-    if (tid < RADIX) {
-        uint32_t sum = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            uint32_t c = s_count[w][tid];
-            s_count[w][tid] = sum;
-            sum += c;
-        }
-        total = sum;
-        tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
-        st_agent(&lookback[tile * RADIX + tid],
-                 (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
-    }
-    const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
-    if (tid < RADIX) {
-        uint32_t prefix = 0;
-        if (tile > 0) {
-            int32_t t = (int32_t)tile - 1;
-            uint32_t spins = 0;
-            while (true) {
-                uint32_t s = ld_acquire_agent(&lookback[(uint32_t)t * RADIX + tid]);
-                if (!(s & MGPU_LB_STATE)) {
-                    if (++spins > kSpinLimit) {
-                        st_agent(a.error_flag, 1u);
+        const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
+        if (tid < RADIX) {
+            uint32_t prefix = 0;
+            if (tile > 0) {
+                int32_t t = (int32_t)tile - 1;
+                uint32_t spins = 0;
+                while (true) {
+                    const uint32_t s = ld_acquire_agent(&job.lookback[(uint32_t)t * RADIX + tid]);
+                    if (!(s & MGPU_LB_STATE)) {
+                        if (++spins > kSpinLimit) {
+                            st_agent(a.error_flag, 1u);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                        continue;
+                    }
+                    prefix += s & MGPU_LB_VALUE;
+                    if (s & MGPU_LB_PREFIX) break;
+                    if (t == 0) {  // tile 0 always publishes a prefix
+                        st_agent(a.error_flag, 3u);
                         break;
                     }
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+                    --t;
                 }
-                prefix += s & MGPU_LB_VALUE;
-                if (s & MGPU_LB_PREFIX) break;
-                if (t == 0) {  // tile 0 always publishes a prefix
-                    st_agent(a.error_flag, 3u);
-                    break;
-                }
-                --t;
+                st_agent(&job.lookback[tile * RADIX + tid], MGPU_LB_PREFIX | (prefix + tile_count));
             }
-            st_agent(&lookback[tile * RADIX + tid], MGPU_LB_PREFIX | (prefix + tile_count));
+            s_global[tid] = job.digit_start[tid] + prefix - local_start;
+#pragma unroll
+            for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
         }
-        s_global[tid] = digit_start[tid] + prefix - local_start;
-#pragma unroll
-        for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // Tile-local position of every key; stage the keys in digit order.
-    uint32_t pos[IPT];
+        // Tile-local position of every key; stage the keys in digit order.
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        pos[j] = s_count[wave][digit_of(j)] + rank[j];
-        s_stage[pos[j]] = key[j];
-    }
-    __syncthreads();
-
-    // Write out in tile order: consecutive threads hold consecutive positions,
-    // so each digit's run of the tile leaves as one contiguous stretch.
-    uint32_t dest[IPT];
-#pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-        const uint32_t p = i * BLOCK + tid;
-        dest[i] = n;
-        if (full || p < tile_valid) {
-            const uint32_t k = s_stage[p];
-            dest[i] = s_global[(k >> shift) & mask] + p;
-            // Offsets outside [0, n) can only come from a broken count or
-            // lookback: flag it instead of writing out of bounds.
-            if (dest[i] < n) keys_out[dest[i]] = k;
-            else st_agent(a.error_flag, 4u);
+        for (int j = 0; j < IPT; ++j) {
+            pos[j] += s_count[wave][digit_of(j)];
+            s_stage[pos[j]] = key[j];
         }
-    }
+        __syncthreads();
 
-    if constexpr (VMODE != MGPU_VALUES_NONE) {
-        const uint32_t nv = VMODE == MGPU_VALUES_MULTI ? a.num_values : 1u;
-        for (uint32_t v = 0; v < nv; ++v) {
-            if constexpr (VMODE == MGPU_VALUES_INDEX) {
+        // The keys are staged: load the next tile's keys into the same
+        // registers, in flight while this tile's keys and values go out.
+        uint32_t next_gtile = 0;
+        TileJob next = job;
+        if constexpr (PERSIST) {
+            next_gtile = __builtin_amdgcn_readfirstlane(s_tile[cur ^ 1]);
+            if (next_gtile < a.num_tiles) {
+                next = make_job<MULTI>(a, next_gtile,
+                                       __builtin_amdgcn_readfirstlane(s_seg[cur ^ 1]));
+                const uint32_t nbase = next.tile * TILE;
+                load_striped<IPT>(next.keys_in, nbase + wave * WAVE_TILE, lane, next.n,
+                                  next.n - nbase >= TILE, 0xffffffffu, key);
+            }
+        }
+
+        // Write out in tile order: consecutive threads hold consecutive
+        // positions, so each digit's run of the tile leaves as one stretch.
+        uint32_t dest[IPT];
+        const int32_t vrem = (int32_t)(tile_valid - tid);  // positions i*BLOCK < vrem are real
 #pragma unroll
-                for (int j = 0; j < IPT; ++j) val[j] = wave_base + j * 64 + lane;
-            } else if constexpr (VMODE == MGPU_VALUES_MULTI) {
-                const uint32_t* src = vals_in(v);
+        for (int i = 0; i < IPT; ++i) {
+            const uint32_t p = i * BLOCK + tid;
+            dest[i] = n;
+            if (full || i * BLOCK < vrem) {
+                const uint32_t k = s_stage[p];
+                dest[i] = s_global[(k >> shift) & mask] + p;
+                // Offsets outside [0, n) can only come from a broken count or
+                // lookback: flag it instead of writing out of bounds.
+                if (dest[i] < n) job.keys_out[dest[i]] = k;
+                else st_agent(a.error_flag, 4u);
+            }
+        }
+
+        if constexpr (VMODE != MGPU_VALUES_NONE) {
+            const uint32_t nv = VMODE == MGPU_VALUES_MULTI ? a.num_values : 1u;
+            for (uint32_t v = 0; v < nv; ++v) {
+                if constexpr (VMODE == MGPU_VALUES_INDEX) {
+                    const uint32_t off = wave_base + lane;
 #pragma unroll
-                for (int j = 0; j < IPT; ++j) {
-                    uint32_t idx = wave_base + j * 64 + lane;
-                    val[j] = (full || idx < n) ? src[idx] : 0u;
+                    for (int j = 0; j < IPT; ++j) val[j] = off + j * 64;
+                } else if constexpr (VMODE == MGPU_VALUES_MULTI) {
+                    load_striped<IPT>(job_vals_in<MULTI>(a, job, v), wave_base, lane, n, full, 0u,
+                                      val);
                 }
-            }
-            __syncthreads();
+                __syncthreads();
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
-            __syncthreads();
-            uint32_t* dst = vals_out(v);
+                for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
+                __syncthreads();
+                uint32_t* dst = job_vals_out<MULTI>(a, job, v);
 #pragma unroll
-            for (int i = 0; i < IPT; ++i) {
-                const uint32_t p = i * BLOCK + tid;
-                if ((full || p < tile_valid) && dest[i] < n) dst[dest[i]] = s_stage[p];
+                for (int i = 0; i < IPT; ++i)
+                    if ((full || i * BLOCK < vrem) && dest[i] < n) dst[dest[i]] = s_stage[i * BLOCK + tid];
             }
+        }
+
+        if constexpr (!PERSIST) {
+            break;
+        } else {
+            if (next_gtile >= a.num_tiles) break;
+            __syncthreads();  // every read of this tile's LDS state is done
+            for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&s_count[0][0])[i] = 0;
+            __syncthreads();
+            job = next;
+            cur ^= 1;
         }
     }
 }
@@ -550,11 +617,19 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
 #define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
-        rank_scatter_body<B, I, R, MODE, false>(a);                                           \
+        rank_scatter_body<B, I, R, MODE, false, false>(a);                                    \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r##R##_p_v##V(mgpuPassArgs a) {                       \
+        rank_scatter_body<B, I, R, MODE, false, true>(a);                                     \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
-        rank_scatter_body<B, I, R, MODE, true>(a);                                            \
+        rank_scatter_body<B, I, R, MODE, true, false>(a);                                     \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_multi_scatter_b##B##_i##I##_r##R##_p_v##V(mgpuMultiPassArgs a) {                 \
+        rank_scatter_body<B, I, R, MODE, true, true>(a);                                      \
     }
 #define MGPU_RANK_SCATTER(B, I, R)                                                            \
     MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
@@ -563,9 +638,7 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
     MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
 
 MGPU_RANK_SCATTER(256, 16, 0)
-MGPU_RANK_SCATTER(256, 24, 0)
 MGPU_RANK_SCATTER(512, 16, 0)
 MGPU_RANK_SCATTER(512, 16, 1)
-MGPU_RANK_SCATTER(512, 24, 0)
 MGPU_RANK_SCATTER(512, 32, 0)
 MGPU_RANK_SCATTER(1024, 16, 0)

@@ -357,8 +357,9 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         else if (P.valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
         else if (P.valueCount == 1) mode = MGPU_VALUES_SINGLE;
         else mode = MGPU_VALUES_MULTI;
-        st = mgpu_launch(e, lane.stream, e->fnMultiPass[mode], P.wideTiles,
-                         e->config.blockThreads, &pa);
+        const uint32_t g = e->config.persistent ? std::min(P.wideTiles, e->multiPassGrid[mode])
+                                                : P.wideTiles;
+        st = mgpu_launch(e, lane.stream, e->fnMultiPass[mode], g, e->config.blockThreads, &pa);
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;

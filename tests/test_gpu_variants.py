@@ -1,0 +1,70 @@
+"""Every built rank_scatter variant (tile shape, in-wave ranking method,
+one-tile-per-workgroup or persistent) gives the same bit-exact result as the
+oracle, for plain and multi-array (segmented large) sorts.  The variant is
+chosen at engine creation through MGPUSORT_TILE / MGPUSORT_RANK /
+MGPUSORT_PERSIST."""
+import os
+
+import numpy as np
+import pytest
+
+import mgpu_old_amd as mg
+from mgpu_old_amd import workloads
+from gpu_helpers import GpuSort, make_engine
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [("256x16", "0", "0"), ("512x16", "0", "0"), ("512x16", "1", "0"),
+            ("512x32", "0", "0"), ("1024x16", "0", "0"), ("256x16", "0", "1"),
+            ("512x16", "0", "1"), ("1024x16", "0", "1")]
+
+
+@pytest.fixture(params=VARIANTS, ids=["_".join(v) for v in VARIANTS])
+def variant_engine(request):
+    tile, rank, persist = request.param
+    saved = {k: os.environ.get(k) for k in ("MGPUSORT_TILE", "MGPUSORT_RANK", "MGPUSORT_PERSIST")}
+    os.environ.update(MGPUSORT_TILE=tile, MGPUSORT_RANK=rank, MGPUSORT_PERSIST=persist)
+    try:
+        eng = make_engine()
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    yield eng
+    eng.close()
+
+
+@pytest.mark.parametrize("n", [1, 5000, 16384, 16385, 1_000_003])
+@pytest.mark.parametrize("vc", [-1, 1, 3])
+def test_variant_matches_oracle(variant_engine, oracle, n, vc):
+    keys = workloads.and3_keys(n, seed=n + vc)
+    vals = [workloads.uniform_keys(n, seed=300 + j) for j in range(max(vc, 0))]
+    g = GpuSort(keys, vals, value_count=vc).run(variant_engine)
+    variant_engine.check_device()
+    rk, rv = oracle.stable_sort(keys, vals, value_count=vc)
+    assert np.array_equal(g.keys(), rk)
+    for o, r in zip(g.values(), rv):
+        assert np.array_equal(o, r)
+
+
+def test_variant_multi_array_path(variant_engine, oracle):
+    """Segments above 2^16 keys take the multi-array device-wide passes."""
+    import torch
+    sizes = [70_000, 3, 200_001, 65_537, 1000]
+    starts = np.zeros(len(sizes) + 1, dtype=np.int64)
+    starts[1:] = np.cumsum(sizes)
+    n = int(starts[-1])
+    keys = workloads.uniform_keys(n, seed=9)
+    vals = [np.arange(n, dtype=np.uint32)]
+    g = GpuSort(keys, vals, end_bit=24)
+    torch.cuda.synchronize()
+    variant_engine.sort_segmented(g.data, starts)
+    variant_engine.check_device()
+    out_k, (out_v,) = g.keys(), g.values()
+    for i in range(len(sizes)):
+        a, b = int(starts[i]), int(starts[i + 1])
+        rk, (rv,) = oracle.stable_sort(keys[a:b], [vals[0][a:b]], first_bit=0, end_bit=24)
+        assert np.array_equal(out_k[a:b], rk) and np.array_equal(out_v[a:b], rv)
+    assert g.data.parity == len(mg.plan_passes(24)) % 2
