@@ -273,6 +273,49 @@ __device__ void multi_scan_body(const mgpuMultiScanArgs& a) {
 // takes them from the tile counter, loading the next tile's keys while the
 // current one is looked back and scattered; otherwise one tile per workgroup.
 
+// Decoupled lookback for digit d of local tile `tile` by a group of G
+// consecutive lanes (k = 0..G-1): each round the group reads the lookback words
+// of G predecessors at once (lane k: tile - 1 - k - round*G), then adds up the
+// counts up to and including the nearest inclusive prefix, or all G of them
+// and goes on.  A round whose needed words are not all published yet is
+// retried.  Tile 0 always publishes a prefix, so the walk ends there at the
+// latest; positions before tile 0 read as an empty prefix and are never
+// counted.  Returns the exclusive prefix of the tile for digit d.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t tile, uint32_t d,
+                                                    uint32_t k, uint32_t lane,
+                                                    uint32_t* error_flag) {
+    if (tile == 0) return 0;
+    constexpr uint32_t GM = (1u << G) - 1u;
+    const uint32_t gfirst = lane & ~(G - 1u);  // the group's first lane in the wave
+    uint32_t prefix = 0, spins = 0;
+    int32_t t0 = (int32_t)tile - 1;  // tile read by lane k = 0 this round
+    while (true) {
+        const int32_t t = t0 - (int32_t)k;
+        const uint32_t s = t >= 0 ? ld_acquire_agent(&lb[(uint32_t)t * RADIX + d]) : MGPU_LB_PREFIX;
+        const uint32_t ready =
+            (uint32_t)(__builtin_amdgcn_ballot_w64((s & MGPU_LB_STATE) != 0) >> gfirst) & GM;
+        const uint32_t pref =
+            (uint32_t)(__builtin_amdgcn_ballot_w64((s & MGPU_LB_PREFIX) != 0) >> gfirst) & GM;
+        // Lanes whose words count: up to the first prefix, or all of them.
+        const uint32_t upto = pref ? ((pref & (0u - pref)) << 1) - 1u : GM;
+        if ((ready & upto) != upto) {
+            if (++spins > kSpinLimit) {
+                st_agent(error_flag, 1u);
+                return prefix;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint32_t v = ((upto >> k) & 1u) ? (s & MGPU_LB_VALUE) : 0u;
+#pragma unroll
+        for (uint32_t off = 1; off < G; off <<= 1) v += __shfl_xor(v, off, G);
+        prefix += v;
+        if (pref) return prefix;
+        t0 -= (int32_t)G;
+    }
+}
+
 // Where one tile reads and writes.
 struct TileJob {
     const uint32_t* keys_in;
@@ -375,6 +418,8 @@ __device__ void rank_scatter_body(const Args& a) {
     __shared__ uint32_t s_tmp[W];
     __shared__ uint32_t s_tile[2];
     __shared__ uint32_t s_seg[2];
+    __shared__ uint32_t s_lstart[RADIX];
+    __shared__ uint32_t s_tcount[RADIX];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
@@ -483,31 +528,22 @@ __device__ void rank_scatter_body(const Args& a) {
         }
         const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
         if (tid < RADIX) {
-            uint32_t prefix = 0;
-            if (tile > 0) {
-                int32_t t = (int32_t)tile - 1;
-                uint32_t spins = 0;
-                while (true) {
-                    const uint32_t s = ld_acquire_agent(&job.lookback[(uint32_t)t * RADIX + tid]);
-                    if (!(s & MGPU_LB_STATE)) {
-                        if (++spins > kSpinLimit) {
-                            st_agent(a.error_flag, 1u);
-                            break;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    prefix += s & MGPU_LB_VALUE;
-                    if (s & MGPU_LB_PREFIX) break;
-                    if (t == 0) {  // tile 0 always publishes a prefix
-                        st_agent(a.error_flag, 3u);
-                        break;
-                    }
-                    --t;
-                }
-                st_agent(&job.lookback[tile * RADIX + tid], MGPU_LB_PREFIX | (prefix + tile_count));
+            s_lstart[tid] = local_start;
+            s_tcount[tid] = tile_count;
+        }
+        __syncthreads();
+        {
+            // BLOCK / RADIX threads per digit walk back over the predecessors.
+            constexpr uint32_t G = BLOCK / RADIX;
+            const uint32_t d = tid / G, k = tid % G;
+            const uint32_t prefix = lookback_window<G>(job.lookback, tile, d, k, lane, a.error_flag);
+            if (k == 0) {
+                if (tile > 0)
+                    st_agent(&job.lookback[tile * RADIX + d], MGPU_LB_PREFIX | (prefix + s_tcount[d]));
+                s_global[d] = job.digit_start[d] + prefix - s_lstart[d];
             }
-            s_global[tid] = job.digit_start[tid] + prefix - local_start;
+        }
+        if (tid < RADIX) {
 #pragma unroll
             for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
         }
