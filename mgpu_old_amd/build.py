@@ -18,7 +18,11 @@ CXX = os.environ.get("CXX", "g++")
 ARCH = "gfx950"
 # Packed FP32 instructions stay off: none of this code wants them, and some of
 # their forms are unreliable on the target boxes.
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                # Work counters are taken by one lane whose reply is used much
+                # later; the atomic optimizer's wave-wide rewrite would wait for
+                # the reply (and every earlier load and store) at once.
+                "-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]
 
 CODE_OBJECTS = {
     "mgpusort_radix.hsaco": ["radixsort.hip"],

@@ -72,8 +72,14 @@ sortStatus_t ensure_lookback(SortLane& L, uint64_t words) {
 
 }  // namespace
 
+const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys) {
+    const uint64_t largeTiles = (keys + e->large.tile() - 1) / e->large.tile();
+    return largeTiles >= (uint64_t)kLargeTilesPerCU * (uint64_t)e->numCUs ? e->large : e->small;
+}
+
 sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes) {
-    const uint32_t tile = e->config.blockThreads * e->config.itemsPerThread;
+    // Enough for either tile shape.
+    const uint32_t tile = std::min(e->small.tile(), e->large.tile());
     const uint64_t tiles = (n + tile - 1) / tile;
     return ensure_lookback(lane, (uint64_t)passes * tiles * MGPU_RADIX);
 }
@@ -191,8 +197,8 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     if (executed) *executed = 0;
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     if (numPasses > MGPU_MAX_PASSES) return SORT_STATUS_INTERNAL_ERROR;
-    const SortConfig& cfg = e->config;
-    const uint32_t tile = cfg.blockThreads * cfg.itemsPerThread;
+    const PassVariant& var = mgpu_pick_variant(e, n);
+    const uint32_t tile = var.tile();
     const uint32_t numTiles = (n + tile - 1) / tile;
     const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
     sortStatus_t st = ensure_lookback(L, lbWords);
@@ -310,8 +316,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         pa.shift = (uint32_t)shiftBits;
         pa.mask = (1u << widths[p]) - 1u;
         pa.num_values = (uint32_t)carried;
-        const uint32_t grid = cfg.persistent ? std::min(numTiles, e->passGrid[mode]) : numTiles;
-        st = mgpu_launch(e, L.stream, e->fnPass[mode], grid, cfg.blockThreads, &pa);
+        st = mgpu_launch(e, L.stream, var.pass[mode], numTiles, var.blockThreads, &pa);
         if (st) return st;
         ++done;
         if (executed) *executed = done;

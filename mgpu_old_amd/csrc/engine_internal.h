@@ -12,13 +12,14 @@
 #include "kernel_args.h"
 #include "segmented_internal.h"
 
-// Variant of the rank_scatter kernel: tile of blockThreads * itemsPerThread
-// keys; in-wave ranking by LDS match masks (0) or per-bit ballots (1).
-struct SortConfig {
-    uint32_t blockThreads;
-    uint32_t itemsPerThread;
-    uint32_t rankMethod;
-    uint32_t persistent;  // workgroups loop over tiles (grid = resident workgroups)
+// A rank_scatter tile shape: blockThreads * itemsPerThread keys per tile, and
+// the kernels of that shape (by value mode, MGPU_VALUES_*).
+struct PassVariant {
+    uint32_t blockThreads = 0;
+    uint32_t itemsPerThread = 0;
+    hipFunction_t pass[4] = {};   // mgpu_rank_scatter_b*_i*_v<mode>
+    hipFunction_t multi[4] = {};  // mgpu_multi_scatter_b*_i*_v<mode>
+    uint32_t tile() const { return blockThreads * itemsPerThread; }
 };
 
 // One in-order queue of sorts: a stream plus the device scratch of the
@@ -54,13 +55,11 @@ struct sortEngine_d {
     hipFunction_t fnCount = nullptr;
     hipFunction_t fnScan = nullptr;
     hipFunction_t fnIota = nullptr;
-    hipFunction_t fnPass[4] = {};  // by value mode (MGPU_VALUES_*)
     hipFunction_t fnMultiCount = nullptr;
     hipFunction_t fnMultiScan = nullptr;
-    hipFunction_t fnMultiPass[4] = {};
-    uint32_t passGrid[4] = {};       // persistent: resident workgroups of fnPass[mode]
-    uint32_t multiPassGrid[4] = {};  // same for fnMultiPass[mode]
-    SortConfig config = {512, 16, 0, 0};
+    // Tile shapes: `large` once a pass has at least kLargeTilesPerCU of its
+    // tiles per CU, `small` below that (more, smaller tiles keep every CU busy).
+    PassVariant small, large;
 
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;
@@ -94,6 +93,11 @@ int mgpu_plan_passes(int numBits, int maxWidth, int* widths);
 
 sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, unsigned grid,
                          unsigned block, void* args);
+
+// A pass over `keys` keys in all uses the large tile shape once it has at least
+// this many tiles per CU (2^23 keys on 256 CUs), else the small one.
+constexpr uint32_t kLargeTilesPerCU = 2;
+const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys);
 
 // Grows the lane's lookback words to cover a device-wide sort of n keys in
 // `passes` passes (a reallocation synchronizes the lane's stream).

@@ -20,34 +20,21 @@ sortStatus_t get_fn(sortEngine_d* e, hipModule_t m, const std::string& name, hip
     return SORT_STATUS_SUCCESS;
 }
 
-// Tile shape and ranking method: MGPUSORT_TILE="<threads>x<items>" and
-// MGPUSORT_RANK=<0|1> pick one of the built variants (for sweeps); otherwise
-// the default.
-SortConfig pick_config() {
-    SortConfig c = {512, 16, 0, 0};
-    const char* t = std::getenv("MGPUSORT_TILE");
-    if (t && *t) {
-        unsigned b = 0, i = 0;
-        if (std::sscanf(t, "%ux%u", &b, &i) == 2 && b && i) {
-            c.blockThreads = b;
-            c.itemsPerThread = i;
-        }
+// Tile shapes: 1024 x 16 for large passes, 256 x 16 for small ones, unless
+// MGPUSORT_TILE="<threads>x<items>" names one built shape for both (sweeps and
+// tests of every shape).
+sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, PassVariant* v) {
+    v->blockThreads = threads;
+    v->itemsPerThread = items;
+    sortStatus_t st = SORT_STATUS_SUCCESS;
+    for (int m = 0; m < 4 && !st; ++m) {
+        char name[96];
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_v%d", threads, items, m);
+        st = get_fn(e, e->radixModule, name, &v->pass[m]);
+        std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_v%d", threads, items, m);
+        if (!st) st = get_fn(e, e->radixModule, name, &v->multi[m]);
     }
-    const char* r = std::getenv("MGPUSORT_RANK");
-    if (r && (*r == '0' || *r == '1')) c.rankMethod = (uint32_t)(*r - '0');
-    const char* ps = std::getenv("MGPUSORT_PERSIST");
-    if (ps && (*ps == '0' || *ps == '1')) c.persistent = (uint32_t)(*ps - '0');
-    return c;
-}
-
-// Workgroups of f that fit on the device at once (grid of a persistent launch).
-uint32_t resident_grid(const sortEngine_d* e, hipFunction_t f) {
-    int blocks = 0;
-    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, (int)e->config.blockThreads,
-                                                           0) != hipSuccess ||
-        blocks < 1)
-        blocks = 1;
-    return (uint32_t)blocks * (uint32_t)e->numCUs;
+    return st;
 }
 
 }  // namespace
@@ -61,26 +48,17 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     sortStatus_t st = get_fn(e, e->radixModule, "mgpu_digit_count", &e->fnCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
-    e->config = pick_config();
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_scan", &e->fnMultiScan);
-    const char* pers = e->config.persistent ? "_p" : "";
-    for (int v = 0; v < 4 && !st; ++v) {
-        char name[96];
-        std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_r%u%s_v%d",
-                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod,
-                      pers, v);
-        st = get_fn(e, e->radixModule, name, &e->fnMultiPass[v]);
-        if (!st) e->multiPassGrid[v] = resident_grid(e, e->fnMultiPass[v]);
+    uint32_t smallShape[2] = {256, 16}, largeShape[2] = {1024, 16};
+    const char* t = std::getenv("MGPUSORT_TILE");
+    unsigned b = 0, i = 0;
+    if (t && std::sscanf(t, "%ux%u", &b, &i) == 2 && b && i) {
+        smallShape[0] = largeShape[0] = b;
+        smallShape[1] = largeShape[1] = i;
     }
-    for (int v = 0; v < 4 && !st; ++v) {
-        char name[96];
-        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u%s_v%d",
-                      e->config.blockThreads, e->config.itemsPerThread, e->config.rankMethod,
-                      pers, v);
-        st = get_fn(e, e->radixModule, name, &e->fnPass[v]);
-        if (!st) e->passGrid[v] = resident_grid(e, e->fnPass[v]);
-    }
+    if (!st) st = load_variant(e, smallShape[0], smallShape[1], &e->small);
+    if (!st) st = load_variant(e, largeShape[0], largeShape[1], &e->large);
     if (st) return st;
     st = mgpu_seg_load(e);
     if (st) return st;

@@ -22,8 +22,7 @@
 // its 64 keys against each other by digit with a match mask (the lanes that
 // hold the same digit), the rank being the number of matching lower lanes
 // plus the wave's running count of that digit.  The match mask is built
-// with LDS atomic ORs (RANK 0) or with one ballot per digit bit (RANK 1); both
-// are built, the engine picks one.
+// with LDS atomic ORs into a per-wave table of 256 lane masks.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -41,7 +40,7 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ __forceinline__ uint32_t ld_acquire_agent(const uint32_t* p) {
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -267,52 +266,77 @@ __device__ void multi_scan_body(const mgpuMultiScanArgs& a) {
 }
 
 // ---------------------------------------------------------------------------
-// rank_scatter: one digit pass, one tile of BLOCK*IPT keys at a time.
+// rank_scatter: one digit pass, one tile of BLOCK*IPT keys per workgroup.
 // MULTI: tiles of several arrays (mgpuMultiPassArgs), else of the single array
-// of mgpuPassArgs.  PERSIST: a workgroup loops over tiles in the order it
-// takes them from the tile counter, loading the next tile's keys while the
-// current one is looked back and scattered; otherwise one tile per workgroup.
+// of mgpuPassArgs.
 
 // Decoupled lookback for digit d of local tile `tile` by a group of G
-// consecutive lanes (k = 0..G-1): each round the group reads the lookback words
-// of G predecessors at once (lane k: tile - 1 - k - round*G), then adds up the
-// counts up to and including the nearest inclusive prefix, or all G of them
-// and goes on.  A round whose needed words are not all published yet is
-// retried.  Tile 0 always publishes a prefix, so the walk ends there at the
-// latest; positions before tile 0 read as an empty prefix and are never
-// counted.  Returns the exclusive prefix of the tile for digit d.
-template <uint32_t G>
+// consecutive lanes (k = 0..G-1), each holding K words: one round reads the
+// lookback words of the G*K nearest predecessors not yet counted (lane k, word
+// q: tile t0 - k*K - q, nearest first), then adds up the counts up to and
+// including the nearest inclusive prefix, or all of them and goes on.  A round
+// whose needed words are not all published yet is read again.  Tile 0 always
+// publishes a prefix, so the walk ends there at the latest; positions before
+// tile 0 read as an empty prefix and are never counted.  Loads are relaxed:
+// a word's count is all that is used of the tile that published it.
+#ifndef MGPU_LOOKBACK_WINDOW
+#define MGPU_LOOKBACK_WINDOW 16
+#endif
+
+__device__ __forceinline__ uint32_t lookback_word(const uint32_t* lb, int32_t t, uint32_t d) {
+    return t >= 0 ? ld_agent(&lb[(uint32_t)t * RADIX + d]) : MGPU_LB_PREFIX;
+}
+
+template <uint32_t K>
+__device__ __forceinline__ void lookback_read(const uint32_t* lb, int32_t t0, uint32_t d,
+                                              uint32_t k, uint32_t (&w)[K]) {
+#pragma unroll
+    for (uint32_t q = 0; q < K; ++q) w[q] = lookback_word(lb, t0 - (int32_t)(k * K + q), d);
+}
+
+// Returns the exclusive prefix of the tile for digit d.  `w` holds the first
+// round's words, read by the caller with lookback_read(lb, tile - 1, ...).
+template <uint32_t G, uint32_t K>
 __device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t tile, uint32_t d,
                                                     uint32_t k, uint32_t lane,
-                                                    uint32_t* error_flag) {
+                                                    uint32_t* error_flag, uint32_t (&w)[K]) {
     if (tile == 0) return 0;
     constexpr uint32_t GM = (1u << G) - 1u;
     const uint32_t gfirst = lane & ~(G - 1u);  // the group's first lane in the wave
     uint32_t prefix = 0, spins = 0;
-    int32_t t0 = (int32_t)tile - 1;  // tile read by lane k = 0 this round
+    int32_t t0 = (int32_t)tile - 1;  // nearest tile of this round
     while (true) {
-        const int32_t t = t0 - (int32_t)k;
-        const uint32_t s = t >= 0 ? ld_acquire_agent(&lb[(uint32_t)t * RADIX + d]) : MGPU_LB_PREFIX;
-        const uint32_t ready =
-            (uint32_t)(__builtin_amdgcn_ballot_w64((s & MGPU_LB_STATE) != 0) >> gfirst) & GM;
-        const uint32_t pref =
-            (uint32_t)(__builtin_amdgcn_ballot_w64((s & MGPU_LB_PREFIX) != 0) >> gfirst) & GM;
-        // Lanes whose words count: up to the first prefix, or all of them.
+        // This lane's words up to its nearest prefix: all published?  Sum.
+        bool ready = true, has_pref = false;
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < K; ++q) {
+            if (!has_pref) {
+                ready = ready && (w[q] & MGPU_LB_STATE) != 0;
+                sum += w[q] & MGPU_LB_VALUE;
+                has_pref = (w[q] & MGPU_LB_PREFIX) != 0;
+            }
+        }
+        const uint32_t rdy = (uint32_t)(__builtin_amdgcn_ballot_w64(ready) >> gfirst) & GM;
+        const uint32_t pref = (uint32_t)(__builtin_amdgcn_ballot_w64(has_pref) >> gfirst) & GM;
+        // Lanes whose words count: up to the first with a prefix, or all.
         const uint32_t upto = pref ? ((pref & (0u - pref)) << 1) - 1u : GM;
-        if ((ready & upto) != upto) {
+        if ((rdy & upto) != upto) {
             if (++spins > kSpinLimit) {
                 st_agent(error_flag, 1u);
                 return prefix;
             }
             __builtin_amdgcn_s_sleep(1);
+            lookback_read<K>(lb, t0, d, k, w);
             continue;
         }
-        uint32_t v = ((upto >> k) & 1u) ? (s & MGPU_LB_VALUE) : 0u;
+        uint32_t v = ((upto >> k) & 1u) ? sum : 0u;
 #pragma unroll
         for (uint32_t off = 1; off < G; off <<= 1) v += __shfl_xor(v, off, G);
         prefix += v;
         if (pref) return prefix;
-        t0 -= (int32_t)G;
+        t0 -= (int32_t)(G * K);
+        lookback_read<K>(lb, t0, d, k, w);
     }
 }
 
@@ -384,40 +408,49 @@ __device__ __forceinline__ uint32_t* job_vals_out(const Args& a, const TileJob& 
 }
 
 // Striped load of one array of a tile: slot j of lane l of wave w holds
-// element tile_base + w*64*IPT + j*64 + l; elements past n read as `fill`.
-// Addresses are one per-lane base plus constant offsets (j * 256 bytes), and
-// the bounds test compares constants against one per-lane remainder, so that
-// nothing per slot stays live across a tile loop.
+// element tile_base + w*64*IPT + j*64 + l.  Raw buffer loads with the array
+// length as the range: the hardware returns 0 for elements at or past n, so a
+// partial tile takes the same branch-free path as a full one (callers never
+// use those slots), and the compiler can wait for each slot separately.  One
+// per-lane byte offset plus constant offsets (j * 256 bytes) address the
+// slots.  Offsets past 2^32 bytes (only past n, in the last tile of the
+// largest arrays) wrap to earlier elements of the same array: still in range,
+// read and ignored.
+constexpr int kBufferDword3 = 0x00020000;  // 32-bit data format, no swizzle
+constexpr int kLoadNontemporal = 2;
+
 template <int IPT>
-__device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t wave_base,
-                                             uint32_t lane, uint32_t n, bool full, uint32_t fill,
-                                             uint32_t* out) {
-    const uint32_t off = wave_base + lane;
-    const uint32_t* p = src + off;
-    if (full) {
+__device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t n, uint32_t wave_base,
+                                             uint32_t lane, uint32_t* out) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<uint32_t*>(src), 0, (int)(n * 4u), kBufferDword3);
+    const uint32_t off = (wave_base + lane) * 4u;
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) out[j] = __builtin_nontemporal_load(p + j * 64);
-    } else {
-        const int32_t rem = (int32_t)(n - off);
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) out[j] = j * 64 < rem ? p[j * 64] : fill;
-    }
+    for (int j = 0; j < IPT; ++j)
+        out[j] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + j * 256u), 0, kLoadNontemporal);
 }
 
-template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool PERSIST, typename Args>
+template <int BLOCK, int IPT, int VMODE, bool MULTI, typename Args>
 __device__ void rank_scatter_body(const Args& a) {
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
     constexpr uint32_t TILE = BLOCK * IPT;
     constexpr uint32_t WAVE_TILE = 64 * IPT;
 
+    // One value array or generated indices are staged beside the keys.  The
+    // in-wave match masks are only live during ranking, before anything is
+    // staged, so they share the value stage's memory.
+    constexpr bool JOINT = VMODE == MGPU_VALUES_SINGLE || VMODE == MGPU_VALUES_INDEX;
+    constexpr uint32_t MATCH_WORDS = (uint32_t)W * RADIX * 2;
+    constexpr uint32_t AUX_WORDS = JOINT && TILE > MATCH_WORDS ? TILE : MATCH_WORDS;
     __shared__ uint32_t s_stage[TILE];
+    __shared__ __attribute__((aligned(16))) uint32_t s_aux[AUX_WORDS];
     __shared__ uint32_t s_count[W][RADIX];
-    __shared__ uint64_t s_match[RANK == 0 ? W : 1][RANK == 0 ? RADIX : 1];
+    uint64_t(*const s_match)[RADIX] = reinterpret_cast<uint64_t(*)[RADIX]>(s_aux);
+    uint32_t* const s_vstage = s_aux;
     __shared__ uint32_t s_global[RADIX];
     __shared__ uint32_t s_tmp[W];
     __shared__ uint32_t s_tile[2];
-    __shared__ uint32_t s_seg[2];
     __shared__ uint32_t s_lstart[RADIX];
     __shared__ uint32_t s_tcount[RADIX];
 
@@ -426,201 +459,169 @@ __device__ void rank_scatter_body(const Args& a) {
     const uint32_t wave = tid >> 6;
     const uint32_t shift = a.shift, mask = a.mask;
 
-    auto acquire = [&](int slot) {  // thread 0 only
+    // The tile comes from the counter in start order, so a tile only ever
+    // waits on tiles that are already running.
+    if (tid == 0) {
         const uint32_t t = __hip_atomic_fetch_add(a.tile_counter, 1u, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT);
-        s_tile[slot] = t;
-        s_seg[slot] = t < a.num_tiles ? find_array<MULTI>(a, t) : 0u;
-    };
-
-    if (tid == 0) acquire(0);
+        s_tile[0] = t;
+        s_tile[1] = t < a.num_tiles ? find_array<MULTI>(a, t) : 0u;
+    }
     for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
         (&s_count[0][0])[i] = 0;
-        if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
+        (&s_match[0][0])[i] = 0;
     }
     __syncthreads();
-    int cur = 0;
     // Tile and array indices are the same for every lane: keep them (and all
     // that is derived from them) in scalar registers.
-    uint32_t gtile = __builtin_amdgcn_readfirstlane(s_tile[0]);
-    if (gtile >= a.num_tiles) {
-        // One workgroup per tile: only a counter left dirty gets here.  A
-        // persistent workgroup just finds the pass already done.
-        if (!PERSIST && tid == 0) st_agent(a.error_flag, 2u);
+    const uint32_t gtile = __builtin_amdgcn_readfirstlane(s_tile[0]);
+    if (gtile >= a.num_tiles) {  // only a counter left dirty gets here
+        if (tid == 0) st_agent(a.error_flag, 2u);
         return;
     }
-    TileJob job = make_job<MULTI>(a, gtile, __builtin_amdgcn_readfirstlane(s_seg[0]));
-    uint32_t key[IPT];
-    {
-        const uint32_t base = job.tile * TILE;
-        load_striped<IPT>(job.keys_in, base + wave * WAVE_TILE, lane, job.n,
-                          job.n - base >= TILE, 0xffffffffu, key);
+    const TileJob job = make_job<MULTI>(a, gtile, __builtin_amdgcn_readfirstlane(s_tile[1]));
+    const uint32_t n = job.n, tile = job.tile;
+    const uint32_t base = tile * TILE;
+    const uint32_t wave_base = base + wave * WAVE_TILE;
+    const uint32_t tile_valid = min(TILE, n - base);
+    const bool full = tile_valid == TILE;
+
+    // Keys and the single value array, striped, requested together: ranking
+    // starts as soon as each key slot is in while the values still load.
+    uint32_t key[IPT], val[IPT];
+    load_striped<IPT>(job.keys_in, n, wave_base, lane, key);
+    if constexpr (VMODE == MGPU_VALUES_SINGLE)
+        load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), n, wave_base, lane, val);
+
+    // Digit of slot j; keys past n get the top digit so that they rank after
+    // every real key of the tile (they are last in input order).
+    const int32_t rem = (int32_t)(n - (wave_base + lane));  // slots j*64 < rem are real
+    auto digit_of = [&](int j) -> uint32_t {
+        if (full) return (key[j] >> shift) & mask;
+        return j * 64 < rem ? ((key[j] >> shift) & mask) : (RADIX - 1);
+    };
+
+    // Rank within the wave: pos[j] = earlier same-digit keys of this wave.
+    // Each lane ORs its bit into the wave's mask for its digit and reads the
+    // mask back; LDS operations of one wave complete in order, so the mask
+    // holds exactly the lanes of this slot with that digit.
+    uint32_t pos[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const uint32_t d = digit_of(j);
+        __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint64_t m =
+            __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t below = lanes_below(m);
+        const uint32_t old = s_count[wave][d];
+        pos[j] = old + below;
+        if (below == 0) {  // lowest lane of the group advances the count
+            s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
+            __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    __syncthreads();
+
+    // Per digit (thread d): wave-exclusive offsets and the tile total,
+    // published at once for the tiles after this one; then the tile-local
+    // digit starts (block scan) are added to the wave offsets.
+    uint32_t total = 0, tile_count = 0;
+    if (tid < RADIX) {
+        uint32_t sum = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            const uint32_t c = s_count[w][tid];
+            s_count[w][tid] = sum;
+            sum += c;
+        }
+        total = sum;
+        tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
+        st_agent(&job.lookback[tile * RADIX + tid],
+                 (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
+    }
+    const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
+    if (tid < RADIX) {
+        s_lstart[tid] = local_start;
+        s_tcount[tid] = tile_count;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
+    }
+    __syncthreads();
+
+    // BLOCK / RADIX threads per digit walk back over the predecessors.  The
+    // first round's words are requested before the staging below so that
+    // their latency overlaps it (starting earlier only waits longer for the
+    // predecessors' own totals).
+    constexpr uint32_t G = BLOCK / RADIX;
+    constexpr uint32_t K = MGPU_LOOKBACK_WINDOW / G > 0 ? MGPU_LOOKBACK_WINDOW / G : 1;
+    const uint32_t lb_digit = tid / G, lb_k = tid % G;
+    uint32_t lb_words[K];
+    lookback_read<K>(job.lookback, (int32_t)tile - 1, lb_digit, lb_k, lb_words);
+
+    // Tile-local position of every key; stage the keys in digit order (and,
+    // with one value array or generated indices, the values beside them, so
+    // that both leave after a single barrier).
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        pos[j] += s_count[wave][digit_of(j)];
+        s_stage[pos[j]] = key[j];
+        if constexpr (JOINT) {
+            if constexpr (VMODE == MGPU_VALUES_INDEX) s_vstage[pos[j]] = wave_base + lane + j * 64;
+            else s_vstage[pos[j]] = val[j];
+        }
     }
 
-    while (true) {
-        if (PERSIST && tid == 0) acquire(cur ^ 1);  // seen after the next barrier
+    {
+        const uint32_t prefix = lookback_window<G, K>(job.lookback, tile, lb_digit, lb_k, lane,
+                                                      a.error_flag, lb_words);
+        if (lb_k == 0) {
+            if (tile > 0)
+                st_agent(&job.lookback[tile * RADIX + lb_digit],
+                         MGPU_LB_PREFIX | (prefix + s_tcount[lb_digit]));
+            s_global[lb_digit] = job.digit_start[lb_digit] + prefix - s_lstart[lb_digit];
+        }
+    }
+    __syncthreads();
 
-        const uint32_t n = job.n, tile = job.tile;
-        const uint32_t base = tile * TILE;
-        const uint32_t wave_base = base + wave * WAVE_TILE;
-        const uint32_t tile_valid = min(TILE, n - base);
-        const bool full = tile_valid == TILE;
-
-        uint32_t val[IPT];
-        if constexpr (VMODE == MGPU_VALUES_SINGLE)
-            load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), wave_base, lane, n, full, 0u, val);
-
-        // Digit of slot j; keys past n get the top digit so that they rank
-        // after every real key of the tile (they are last in input order).
-        const int32_t rem = (int32_t)(n - (wave_base + lane));  // slots j*64 < rem are real
-        auto digit_of = [&](int j) -> uint32_t {
-            if (full) return (key[j] >> shift) & mask;
-            return j * 64 < rem ? ((key[j] >> shift) & mask) : (RADIX - 1);
-        };
-
-        // Rank within the wave: rank[j] = earlier same-digit keys of this wave.
-        uint32_t pos[IPT];
+    // Write out in tile order: consecutive threads hold consecutive positions,
+    // so each digit's run of the tile leaves as one stretch.
+    constexpr bool KEEP_DEST = VMODE == MGPU_VALUES_MULTI;
+    uint32_t dest[KEEP_DEST ? IPT : 1];
+    const int32_t vrem = (int32_t)(tile_valid - tid);  // positions i*BLOCK < vrem are real
+    uint32_t* const vout0 = JOINT ? job_vals_out<MULTI>(a, job, 0) : nullptr;
 #pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            const uint32_t d = digit_of(j);
-            uint64_t m;
-            if constexpr (RANK == 1) {
-                m = ~0ull;
-#pragma unroll
-                for (int b = 0; b < MGPU_RADIX_BITS; ++b) {
-                    const uint32_t bit = (d >> b) & 1u;
-                    const uint64_t bb = __builtin_amdgcn_ballot_w64(bit != 0);
-                    m &= bit ? bb : ~bb;
-                }
+    for (int i = 0; i < IPT; ++i) {
+        const uint32_t p = i * BLOCK + tid;
+        uint32_t dst = n;
+        if (full || i * BLOCK < vrem) {
+            const uint32_t k = s_stage[p];
+            dst = s_global[(k >> shift) & mask] + p;
+            // Offsets outside [0, n) can only come from a broken count or
+            // lookback: flag it instead of writing out of bounds.
+            if (dst < n) {
+                job.keys_out[dst] = k;
+                if constexpr (JOINT) vout0[dst] = s_vstage[p];
             } else {
-                __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-                m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
-            const uint32_t below = lanes_below(m);
-            const uint32_t old = s_count[wave][d];
-            pos[j] = old + below;
-            if (below == 0) {  // lowest lane of the group advances the count
-                s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
-                if constexpr (RANK == 0)
-                    __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                st_agent(a.error_flag, 4u);
             }
         }
-        __syncthreads();
+        if constexpr (KEEP_DEST) dest[i] = dst;
+    }
 
-        // Per digit (thread d): wave-exclusive offsets and the tile total,
-        // published at once for the tiles after this one.
-        uint32_t total = 0, tile_count = 0;
-        if (tid < RADIX) {
-            uint32_t sum = 0;
-#pragma unroll
-            for (int w = 0; w < W; ++w) {
-                const uint32_t c = s_count[w][tid];
-                s_count[w][tid] = sum;
-                sum += c;
-            }
-            total = sum;
-            tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
-            st_agent(&job.lookback[tile * RADIX + tid],
-                     (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
-        }
-        const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
-        if (tid < RADIX) {
-            s_lstart[tid] = local_start;
-            s_tcount[tid] = tile_count;
-        }
-        __syncthreads();
-        {
-            // BLOCK / RADIX threads per digit walk back over the predecessors.
-            constexpr uint32_t G = BLOCK / RADIX;
-            const uint32_t d = tid / G, k = tid % G;
-            const uint32_t prefix = lookback_window<G>(job.lookback, tile, d, k, lane, a.error_flag);
-            if (k == 0) {
-                if (tile > 0)
-                    st_agent(&job.lookback[tile * RADIX + d], MGPU_LB_PREFIX | (prefix + s_tcount[d]));
-                s_global[d] = job.digit_start[d] + prefix - s_lstart[d];
-            }
-        }
-        if (tid < RADIX) {
-#pragma unroll
-            for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
-        }
-        __syncthreads();
-
-        // Tile-local position of every key; stage the keys in digit order.
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            pos[j] += s_count[wave][digit_of(j)];
-            s_stage[pos[j]] = key[j];
-        }
-        __syncthreads();
-
-        // The keys are staged: load the next tile's keys into the same
-        // registers, in flight while this tile's keys and values go out.
-        uint32_t next_gtile = 0;
-        TileJob next = job;
-        if constexpr (PERSIST) {
-            next_gtile = __builtin_amdgcn_readfirstlane(s_tile[cur ^ 1]);
-            if (next_gtile < a.num_tiles) {
-                next = make_job<MULTI>(a, next_gtile,
-                                       __builtin_amdgcn_readfirstlane(s_seg[cur ^ 1]));
-                const uint32_t nbase = next.tile * TILE;
-                load_striped<IPT>(next.keys_in, nbase + wave * WAVE_TILE, lane, next.n,
-                                  next.n - nbase >= TILE, 0xffffffffu, key);
-            }
-        }
-
-        // Write out in tile order: consecutive threads hold consecutive
-        // positions, so each digit's run of the tile leaves as one stretch.
-        uint32_t dest[IPT];
-        const int32_t vrem = (int32_t)(tile_valid - tid);  // positions i*BLOCK < vrem are real
-#pragma unroll
-        for (int i = 0; i < IPT; ++i) {
-            const uint32_t p = i * BLOCK + tid;
-            dest[i] = n;
-            if (full || i * BLOCK < vrem) {
-                const uint32_t k = s_stage[p];
-                dest[i] = s_global[(k >> shift) & mask] + p;
-                // Offsets outside [0, n) can only come from a broken count or
-                // lookback: flag it instead of writing out of bounds.
-                if (dest[i] < n) job.keys_out[dest[i]] = k;
-                else st_agent(a.error_flag, 4u);
-            }
-        }
-
-        if constexpr (VMODE != MGPU_VALUES_NONE) {
-            const uint32_t nv = VMODE == MGPU_VALUES_MULTI ? a.num_values : 1u;
-            for (uint32_t v = 0; v < nv; ++v) {
-                if constexpr (VMODE == MGPU_VALUES_INDEX) {
-                    const uint32_t off = wave_base + lane;
-#pragma unroll
-                    for (int j = 0; j < IPT; ++j) val[j] = off + j * 64;
-                } else if constexpr (VMODE == MGPU_VALUES_MULTI) {
-                    load_striped<IPT>(job_vals_in<MULTI>(a, job, v), wave_base, lane, n, full, 0u,
-                                      val);
-                }
-                __syncthreads();
-#pragma unroll
-                for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
-                __syncthreads();
-                uint32_t* dst = job_vals_out<MULTI>(a, job, v);
-#pragma unroll
-                for (int i = 0; i < IPT; ++i)
-                    if ((full || i * BLOCK < vrem) && dest[i] < n) dst[dest[i]] = s_stage[i * BLOCK + tid];
-            }
-        }
-
-        if constexpr (!PERSIST) {
-            break;
-        } else {
-            if (next_gtile >= a.num_tiles) break;
-            __syncthreads();  // every read of this tile's LDS state is done
-            for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&s_count[0][0])[i] = 0;
+    // Two or more value arrays: one after the other through the key stage.
+    if constexpr (VMODE == MGPU_VALUES_MULTI) {
+        for (uint32_t v = 0; v < a.num_values; ++v) {
+            load_striped<IPT>(job_vals_in<MULTI>(a, job, v), n, wave_base, lane, val);
             __syncthreads();
-            job = next;
-            cur ^= 1;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
+            __syncthreads();
+            uint32_t* dstv = job_vals_out<MULTI>(a, job, v);
+#pragma unroll
+            for (int i = 0; i < IPT; ++i)
+                if ((full || i * BLOCK < vrem) && dest[i] < n) dstv[dest[i]] = s_stage[i * BLOCK + tid];
         }
     }
 }
@@ -650,31 +651,23 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
     multi_scan_body(a);
 }
 
-#define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
+#define MGPU_RANK_SCATTER_MODE(B, I, V, MODE)                                                \
     extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
-        rank_scatter_body<B, I, R, MODE, false, false>(a);                                    \
+        mgpu_rank_scatter_b##B##_i##I##_v##V(mgpuPassArgs a) {                                \
+        rank_scatter_body<B, I, MODE, false>(a);                                              \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_r##R##_p_v##V(mgpuPassArgs a) {                       \
-        rank_scatter_body<B, I, R, MODE, false, true>(a);                                     \
-    }                                                                                         \
-    extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
-        rank_scatter_body<B, I, R, MODE, true, false>(a);                                     \
-    }                                                                                         \
-    extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_multi_scatter_b##B##_i##I##_r##R##_p_v##V(mgpuMultiPassArgs a) {                 \
-        rank_scatter_body<B, I, R, MODE, true, true>(a);                                      \
+        mgpu_multi_scatter_b##B##_i##I##_v##V(mgpuMultiPassArgs a) {                          \
+        rank_scatter_body<B, I, MODE, true>(a);                                               \
     }
-#define MGPU_RANK_SCATTER(B, I, R)                                                            \
-    MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
-    MGPU_RANK_SCATTER_MODE(B, I, R, 1, MGPU_VALUES_INDEX)                                     \
-    MGPU_RANK_SCATTER_MODE(B, I, R, 2, MGPU_VALUES_SINGLE)                                    \
-    MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
+#define MGPU_RANK_SCATTER(B, I)                                                               \
+    MGPU_RANK_SCATTER_MODE(B, I, 0, MGPU_VALUES_NONE)                                         \
+    MGPU_RANK_SCATTER_MODE(B, I, 1, MGPU_VALUES_INDEX)                                        \
+    MGPU_RANK_SCATTER_MODE(B, I, 2, MGPU_VALUES_SINGLE)                                       \
+    MGPU_RANK_SCATTER_MODE(B, I, 3, MGPU_VALUES_MULTI)
 
-MGPU_RANK_SCATTER(256, 16, 0)
-MGPU_RANK_SCATTER(512, 16, 0)
-MGPU_RANK_SCATTER(512, 16, 1)
-MGPU_RANK_SCATTER(512, 32, 0)
-MGPU_RANK_SCATTER(1024, 16, 0)
+// Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest for
+// large arrays), 512 x 16 and 256 x 16 (more tiles for smaller arrays).
+MGPU_RANK_SCATTER(256, 16)
+MGPU_RANK_SCATTER(512, 16)
+MGPU_RANK_SCATTER(1024, 16)

@@ -183,7 +183,10 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     std::vector<mgpuSegDesc> wideDescs;
     std::vector<uint32_t> tileBegin, chunks;
     if (!P.wide.empty()) {
-        const uint32_t tile = e->config.blockThreads * e->config.itemsPerThread;
+        uint64_t wideKeys = 0;
+        for (uint32_t k : P.wide) wideKeys += segs[k].n;
+        P.wideVariant = &mgpu_pick_variant(e, wideKeys);
+        const uint32_t tile = P.wideVariant->tile();
         tileBegin.push_back(0);
         for (size_t k = 0; k < P.wide.size(); ++k) {
             const mgpuSegDesc& d = segs[P.wide[k]];
@@ -357,9 +360,8 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         else if (P.valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
         else if (P.valueCount == 1) mode = MGPU_VALUES_SINGLE;
         else mode = MGPU_VALUES_MULTI;
-        const uint32_t g = e->config.persistent ? std::min(P.wideTiles, e->multiPassGrid[mode])
-                                                : P.wideTiles;
-        st = mgpu_launch(e, lane.stream, e->fnMultiPass[mode], g, e->config.blockThreads, &pa);
+        st = mgpu_launch(e, lane.stream, P.wideVariant->multi[mode], P.wideTiles,
+                         P.wideVariant->blockThreads, &pa);
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;

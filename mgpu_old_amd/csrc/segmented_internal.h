@@ -10,6 +10,7 @@
 
 struct sortEngine_d;
 struct SortLane;
+struct PassVariant;
 
 // Kernel variants of mgpusort_seg.hsaco: radix bits 4..8 x items per thread
 // {8, 16}, named mgpu_segsort_r<bits>_i<items>.
@@ -63,6 +64,7 @@ struct SegPlanData {
     std::vector<mgpuSegDesc> hostSegs;
     std::vector<SegLaunch> launches;
     std::vector<uint32_t> wide;  // segments sorted by the device-wide path
+    const PassVariant* wideVariant = nullptr;  // tile shape of the device-wide passes
     uint32_t wideTiles = 0;      // tiles of all device-wide segments
     uint32_t wideChunks = 0;     // digit_count chunks of them
     mgpuSegDesc* devWide = nullptr;

@@ -1,8 +1,6 @@
-"""Every built rank_scatter variant (tile shape, in-wave ranking method,
-one-tile-per-workgroup or persistent) gives the same bit-exact result as the
-oracle, for plain and multi-array (segmented large) sorts.  The variant is
-chosen at engine creation through MGPUSORT_TILE / MGPUSORT_RANK /
-MGPUSORT_PERSIST."""
+"""Every built rank_scatter tile shape gives the same bit-exact result as the
+oracle, for plain and multi-array (segmented large) sorts.  MGPUSORT_TILE,
+read at engine creation, forces one shape for every size."""
 import os
 
 import numpy as np
@@ -14,24 +12,20 @@ from gpu_helpers import GpuSort, make_engine
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [("256x16", "0", "0"), ("512x16", "0", "0"), ("512x16", "1", "0"),
-            ("512x32", "0", "0"), ("1024x16", "0", "0"), ("256x16", "0", "1"),
-            ("512x16", "0", "1"), ("1024x16", "0", "1")]
+TILES = ["256x16", "512x16", "1024x16"]
 
 
-@pytest.fixture(params=VARIANTS, ids=["_".join(v) for v in VARIANTS])
+@pytest.fixture(params=TILES)
 def variant_engine(request):
-    tile, rank, persist = request.param
-    saved = {k: os.environ.get(k) for k in ("MGPUSORT_TILE", "MGPUSORT_RANK", "MGPUSORT_PERSIST")}
-    os.environ.update(MGPUSORT_TILE=tile, MGPUSORT_RANK=rank, MGPUSORT_PERSIST=persist)
+    saved = os.environ.get("MGPUSORT_TILE")
+    os.environ["MGPUSORT_TILE"] = request.param
     try:
         eng = make_engine()
     finally:
-        for k, v in saved.items():
-            if v is None:
-                os.environ.pop(k, None)
-            else:
-                os.environ[k] = v
+        if saved is None:
+            os.environ.pop("MGPUSORT_TILE", None)
+        else:
+            os.environ["MGPUSORT_TILE"] = saved
     yield eng
     eng.close()
 
