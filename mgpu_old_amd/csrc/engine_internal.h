@@ -95,8 +95,9 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
                          unsigned block, void* args);
 
 // A pass over `keys` keys in all uses the large tile shape once it has at least
-// this many tiles per CU (2^23 keys on 256 CUs), else the small one.
-constexpr uint32_t kLargeTilesPerCU = 2;
+// this many tiles per CU (2^22 keys on 256 CUs), else the small one (measured:
+// 1024 x 16 is ahead from 2^22 keys up, 512 x 16 below).
+constexpr uint32_t kLargeTilesPerCU = 1;
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys);
 
 // Grows the lane's lookback words to cover a device-wide sort of n keys in

@@ -20,7 +20,7 @@ sortStatus_t get_fn(sortEngine_d* e, hipModule_t m, const std::string& name, hip
     return SORT_STATUS_SUCCESS;
 }
 
-// Tile shapes: 1024 x 16 for large passes, 256 x 16 for small ones, unless
+// Tile shapes: 1024 x 16 for large passes, 512 x 16 for small ones, unless
 // MGPUSORT_TILE="<threads>x<items>" names one built shape for both (sweeps and
 // tests of every shape).
 sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, PassVariant* v) {
@@ -50,7 +50,7 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_scan", &e->fnMultiScan);
-    uint32_t smallShape[2] = {256, 16}, largeShape[2] = {1024, 16};
+    uint32_t smallShape[2] = {512, 16}, largeShape[2] = {1024, 16};
     const char* t = std::getenv("MGPUSORT_TILE");
     unsigned b = 0, i = 0;
     if (t && std::sscanf(t, "%ux%u", &b, &i) == 2 && b && i) {

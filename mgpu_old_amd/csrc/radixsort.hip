@@ -666,8 +666,8 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
     MGPU_RANK_SCATTER_MODE(B, I, 2, MGPU_VALUES_SINGLE)                                       \
     MGPU_RANK_SCATTER_MODE(B, I, 3, MGPU_VALUES_MULTI)
 
-// Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest for
-// large arrays), 512 x 16 and 256 x 16 (more tiles for smaller arrays).
+// Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest from
+// 2^22 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps).
 MGPU_RANK_SCATTER(256, 16)
 MGPU_RANK_SCATTER(512, 16)
 MGPU_RANK_SCATTER(1024, 16)

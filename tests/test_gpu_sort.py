@@ -77,6 +77,19 @@ def test_sizes_and_value_counts(engine, oracle, n, vc):
     _check_vs_oracle(oracle, keys, g, vals, vc)
 
 
+@pytest.mark.parametrize("n", [255 * 16384, 255 * 16384 + 1])
+@pytest.mark.parametrize("vc", [-1, 1])
+def test_tile_shape_switch(engine, oracle, n, vc):
+    """The engine takes the 512 x 16 tiles below 256 tiles of 1024 x 16 keys
+    (one per CU on 256 CUs) and the 1024 x 16 tiles from there on: both sides
+    of the switch against the oracle."""
+    keys = workloads.uniform_keys(n, seed=n)
+    vals = [workloads.uniform_keys(n, seed=n + 1)] if vc == 1 else []
+    g = GpuSort(keys, vals, value_count=vc).run(engine)
+    engine.check_device()
+    _check_vs_oracle(oracle, keys, g, vals, value_count=vc)
+
+
 @pytest.mark.parametrize("vc", [3, 4, 5, 6])
 def test_multi_value_arrays(engine, oracle, vc):
     n = 123_457
