@@ -30,6 +30,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+# HBM bytes per launch of the dominant kernel, measured by rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE passes over this script and scaled by the same
+# counters' reading of a plain 1 GiB copy (the profile summary beside it
+# gives the commands); used only when the kernel and size match.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rank_scatter_2p28_kv_traffic.json")
 
 
 def parse():
@@ -388,6 +393,14 @@ def main():
             "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per_launch / avg_s / HBM_PEAK,
             "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": launches,
             "bytes_per_launch": per_launch}
+    try:
+        with open(TRAFFIC_FILE) as f:
+            measured = json.load(f)
+        if measured.get("kernel") == name and measured.get("n") == n:
+            roof["traffic"] = measured["traffic_bytes"]
+            roof["traffic_source"] = measured["source"]
+    except (OSError, ValueError, KeyError):
+        pass
     kernel_ms = {k: {"avg_ms": v[1] / v[0], "launches": v[0]} for k, v in kt.items()}
 
     extra = []
