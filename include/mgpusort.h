@@ -288,6 +288,22 @@ sortStatus_t SORTAPI sortServerDestroy(sortServer_t server);
  * lookback wait that gave up) since the last call; SUCCESS otherwise. */
 sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine);
 
+/* What the engine chose at creation, and the last device inconsistency code
+ * (0 none; 1 a lookback wait gave up; 2 a tile counter left dirty; 4 a
+ * scatter offset out of range) without clearing it.  No counterpart in the
+ * reference. */
+typedef struct {
+    int device;
+    int numCUs;
+    int reserved0;
+    int rankMethod;    /* 1: one LDS atomic add per key; 0: LDS match masks */
+    int smallTileKeys; /* tile of passes below the switch */
+    int largeTileKeys; /* tile of passes from the switch on */
+    unsigned int deviceError;
+    int reserved;
+} sortEngineInfo_t;
+sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info);
+
 /* Per-kernel timing: while enabled, every launch is bracketed by hipEvents
  * on the engine's stream.  sortReadTiming waits for the stream, fills up to
  * maxEntries aggregates (one per kernel name, in first-launch order), clears

@@ -111,6 +111,21 @@ class KernelTiming(ctypes.Structure):
     ]
 
 
+class EngineInfo(ctypes.Structure):
+    """sortEngineInfo_t: what the engine chose at creation, and the last
+    device inconsistency code (not cleared)."""
+    _fields_ = [
+        ("device", ctypes.c_int),
+        ("numCUs", ctypes.c_int),
+        ("reserved0", ctypes.c_int),
+        ("rankMethod", ctypes.c_int),
+        ("smallTileKeys", ctypes.c_int),
+        ("largeTileKeys", ctypes.c_int),
+        ("deviceError", ctypes.c_uint),
+        ("reserved", ctypes.c_int),
+    ]
+
+
 _lib = None
 
 
@@ -164,6 +179,7 @@ def lib():
                                                 P(SortCompletion)]),
             "sortServerDestroy": (ctypes.c_int, [ctypes.c_void_p]),
             "sortCheckDevice": (ctypes.c_int, [eng]),
+            "sortEngineInfo": (ctypes.c_int, [eng, P(EngineInfo)]),
             "sortEnableTiming": (ctypes.c_int, [eng, ctypes.c_bool]),
             "sortReadTiming": (ctypes.c_int, [eng, P(KernelTiming), ctypes.c_int]),
         }
@@ -236,6 +252,13 @@ class Engine:
     def check_device(self):
         """Raise if a kernel flagged an inconsistency since the last check."""
         _check(lib().sortCheckDevice(self._h), "sortCheckDevice")
+
+    def info(self):
+        """sortEngineInfo as a dict."""
+        out = EngineInfo()
+        _check(lib().sortEngineInfo(self._h, ctypes.byref(out)), "sortEngineInfo")
+        return {name: getattr(out, name) for name, _ in EngineInfo._fields_
+                if not name.startswith("reserved")}
 
     def enable_timing(self, enable=True):
         _check(lib().sortEnableTiming(self._h, bool(enable)), "sortEnableTiming")

@@ -472,6 +472,22 @@ sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine) {
     return SORT_STATUS_KERNEL_ERROR;
 }
 
+sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info) {
+    if (!engine || !info) return SORT_STATUS_INVALID_VALUE;
+    std::memset(info, 0, sizeof(*info));
+    info->device = engine->device;
+    info->numCUs = engine->numCUs;
+    info->rankMethod = (int)engine->rankMethod;
+    info->smallTileKeys = (int)engine->small.tile();
+    info->largeTileKeys = (int)engine->large.tile();
+    if (hipStreamSynchronize(engine->lane.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    uint32_t flag = 0;
+    if (hipMemcpy(&flag, engine->errorFlag, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    info->deviceError = flag;
+    return SORT_STATUS_SUCCESS;
+}
+
 sortStatus_t SORTAPI sortEnableTiming(sortEngine_t engine, bool enable) {
     if (!engine) return SORT_STATUS_INVALID_VALUE;
     engine->timing = enable;

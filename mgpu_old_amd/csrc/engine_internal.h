@@ -60,6 +60,7 @@ struct sortEngine_d {
     // Tile shapes: `large` once a pass has at least kLargeTilesPerCU of its
     // tiles per CU, `small` below that (more, smaller tiles keep every CU busy).
     PassVariant small, large;
+    uint32_t rankMethod = 0;  // 1: atomic-add ranking (checked at creation), 0: match masks
 
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;

@@ -22,6 +22,34 @@
 #define MGPU_VALUES_SINGLE 2  // carry vals_in[0]
 #define MGPU_VALUES_MULTI 3   // carry vals_in[0 .. num_values)
 
+// Digit that lane `lane` of wave `wave` uses in round `round` of
+// mgpu_rank_probe (host and device share it): per wave a different pattern,
+// from every lane on one digit to every lane on its own.
+#if defined(__HIPCC__)
+#define MGPU_HOST_DEVICE __host__ __device__
+#else
+#define MGPU_HOST_DEVICE
+#endif
+static inline MGPU_HOST_DEVICE uint32_t mgpu_rank_probe_digit(uint32_t wave, uint32_t round,
+                                                               uint32_t lane) {
+    uint32_t h = wave * 0x9E3779B9u + round * 0x85EBCA6Bu + lane * 0xC2B2AE35u + 1u;
+    h ^= h >> 16;
+    h *= 0x85EBCA6Bu;
+    h ^= h >> 13;
+    h *= 0xC2B2AE35u;
+    h ^= h >> 16;
+    if (wave <= 8) return h & ((1u << wave) - 1u);  // 1, 2, 4, ... 256 distinct digits
+    switch (wave) {
+        case 9: return (h & (h >> 8) & (h >> 16)) & 255u;  // skewed toward 0
+        case 10: return lane >> 3;                         // runs of 8 lanes
+        case 11: return 63u - lane;                        // all distinct, descending
+        case 12: return lane & 1u;                         // alternating
+        case 13: return (lane * 37u) & 255u;               // all distinct, scattered
+        case 14: return h & 3u;
+        default: return 255u - (h & 1u);
+    }
+}
+
 // digit_count: histograms of every pass's digit over all keys in one read.
 struct mgpuCountArgs {
     const uint32_t* keys;

@@ -19,10 +19,12 @@
 // Stable ranking inside a tile (wave64): keys are loaded striped, lane l of
 // wave w holding key  tile_base + w*64*IPT + j*64 + l  in slot j, so the
 // order (wave, slot, lane) is the input order.  For each slot the wave ranks
-// its 64 keys against each other by digit with a match mask (the lanes that
-// hold the same digit), the rank being the number of matching lower lanes
-// plus the wave's running count of that digit.  The match mask is built
-// with LDS atomic ORs into a per-wave table of 256 lane masks.
+// its 64 keys by digit: a key's rank is the number of lower lanes of the slot
+// with the same digit plus the wave's running count of that digit.  RANK 0
+// builds a match mask of those lanes with LDS atomic ORs into a per-wave
+// table of 256 lane masks; RANK 1 takes the rank straight from one LDS atomic
+// add per key on the wave's count, where the device serves same-word lanes in
+// ascending order (checked by the engine at creation).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -430,7 +432,13 @@ __device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t n, ui
         out[j] = __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off + j * 256u), 0, kLoadNontemporal);
 }
 
-template <int BLOCK, int IPT, int VMODE, bool MULTI, typename Args>
+// RANK 0: in-wave ranking by LDS match masks (documented behaviour only).
+// RANK 1: one LDS atomic add per key on the wave's count of its digit: the
+// returned old count is the key's stable rank when the lanes of one
+// instruction that hit the same count are served in ascending lane order,
+// which the engine checks at creation (mgpu_rank_probe) before it picks
+// these kernels.
+template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, typename Args>
 __device__ void rank_scatter_body(const Args& a) {
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
@@ -441,8 +449,9 @@ __device__ void rank_scatter_body(const Args& a) {
     // in-wave match masks are only live during ranking, before anything is
     // staged, so they share the value stage's memory.
     constexpr bool JOINT = VMODE == MGPU_VALUES_SINGLE || VMODE == MGPU_VALUES_INDEX;
-    constexpr uint32_t MATCH_WORDS = (uint32_t)W * RADIX * 2;
-    constexpr uint32_t AUX_WORDS = JOINT && TILE > MATCH_WORDS ? TILE : MATCH_WORDS;
+    constexpr uint32_t MATCH_WORDS = RANK == 0 ? (uint32_t)W * RADIX * 2 : 0;
+    constexpr uint32_t AUX_WORDS0 = JOINT && TILE > MATCH_WORDS ? TILE : MATCH_WORDS;
+    constexpr uint32_t AUX_WORDS = AUX_WORDS0 > 0 ? AUX_WORDS0 : 2;
     __shared__ uint32_t s_stage[TILE];
     __shared__ __attribute__((aligned(16))) uint32_t s_aux[AUX_WORDS];
     __shared__ uint32_t s_count[W][RADIX];
@@ -469,7 +478,7 @@ __device__ void rank_scatter_body(const Args& a) {
     }
     for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
         (&s_count[0][0])[i] = 0;
-        (&s_match[0][0])[i] = 0;
+        if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
     }
     __syncthreads();
     // Tile and array indices are the same for every lane: keep them (and all
@@ -502,24 +511,30 @@ __device__ void rank_scatter_body(const Args& a) {
     };
 
     // Rank within the wave: pos[j] = earlier same-digit keys of this wave.
-    // Each lane ORs its bit into the wave's mask for its digit and reads the
-    // mask back; LDS operations of one wave complete in order, so the mask
-    // holds exactly the lanes of this slot with that digit.
     uint32_t pos[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const uint32_t d = digit_of(j);
-        __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint64_t m =
-            __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t below = lanes_below(m);
-        const uint32_t old = s_count[wave][d];
-        pos[j] = old + below;
-        if (below == 0) {  // lowest lane of the group advances the count
-            s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
-            __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        if constexpr (RANK == 1) {
+            pos[j] = __hip_atomic_fetch_add(&s_count[wave][d], 1u, __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            // Each lane ORs its bit into the wave's mask for its digit and
+            // reads the mask back; LDS operations of one wave complete in
+            // order, so the mask holds exactly the lanes of this slot with
+            // that digit.
+            __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint64_t m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+            const uint32_t below = lanes_below(m);
+            const uint32_t old = s_count[wave][d];
+            pos[j] = old + below;
+            if (below == 0) {  // lowest lane of the group advances the count
+                s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
+                __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         }
     }
     __syncthreads();
@@ -651,23 +666,45 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
     multi_scan_body(a);
 }
 
-#define MGPU_RANK_SCATTER_MODE(B, I, V, MODE)                                                \
+#define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
     extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_rank_scatter_b##B##_i##I##_v##V(mgpuPassArgs a) {                                \
-        rank_scatter_body<B, I, MODE, false>(a);                                              \
+        mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
+        rank_scatter_body<B, I, R, MODE, false>(a);                                           \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
-        mgpu_multi_scatter_b##B##_i##I##_v##V(mgpuMultiPassArgs a) {                          \
-        rank_scatter_body<B, I, MODE, true>(a);                                               \
+        mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
+        rank_scatter_body<B, I, R, MODE, true>(a);                                            \
     }
-#define MGPU_RANK_SCATTER(B, I)                                                               \
-    MGPU_RANK_SCATTER_MODE(B, I, 0, MGPU_VALUES_NONE)                                         \
-    MGPU_RANK_SCATTER_MODE(B, I, 1, MGPU_VALUES_INDEX)                                        \
-    MGPU_RANK_SCATTER_MODE(B, I, 2, MGPU_VALUES_SINGLE)                                       \
-    MGPU_RANK_SCATTER_MODE(B, I, 3, MGPU_VALUES_MULTI)
+#define MGPU_RANK_SCATTER(B, I, R)                                                            \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 1, MGPU_VALUES_INDEX)                                     \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 2, MGPU_VALUES_SINGLE)                                    \
+    MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
 
 // Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest from
-// 2^22 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps).
-MGPU_RANK_SCATTER(256, 16)
-MGPU_RANK_SCATTER(512, 16)
-MGPU_RANK_SCATTER(1024, 16)
+// 2^22 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps);
+// each with both ranking methods.
+MGPU_RANK_SCATTER(256, 16, 0)
+MGPU_RANK_SCATTER(512, 16, 0)
+MGPU_RANK_SCATTER(1024, 16, 0)
+MGPU_RANK_SCATTER(256, 16, 1)
+MGPU_RANK_SCATTER(512, 16, 1)
+MGPU_RANK_SCATTER(1024, 16, 1)
+
+// Check of the ordering RANK 1 relies on: wave w of one 1024-thread
+// workgroup adds 1 per lane, for 16 rounds, to the count of digit
+// mgpu_rank_probe_digit(w, round, lane) and writes back the old count it was
+// given; the host compares with the counts in ascending lane order.  Digit
+// patterns range from all lanes on one count to all on different ones.
+extern "C" __global__ void __launch_bounds__(1024) mgpu_rank_probe(uint32_t* out) {
+    __shared__ uint32_t cnt[16][RADIX];
+    const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (uint32_t i = tid; i < 16 * RADIX; i += 1024) (&cnt[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll 1
+    for (uint32_t r = 0; r < 16; ++r) {
+        const uint32_t d = mgpu_rank_probe_digit(wave, r, lane);
+        out[r * 1024 + tid] = __hip_atomic_fetch_add(&cnt[wave][d], 1u, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}

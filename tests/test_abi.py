@@ -193,4 +193,4 @@ def test_code_objects_built_for_gfx950():
         data = open(path, "rb").read()
         assert data[:4] == b"\x7fELF"
         assert b"gfx950" in data
-        assert b"mgpu_digit_count" in data and b"mgpu_rank_scatter_b1024_i16_v0" in data
+        assert b"mgpu_digit_count" in data and b"mgpu_rank_scatter_b1024_i16_r1_v0" in data

@@ -1,6 +1,9 @@
-"""Every built rank_scatter tile shape gives the same bit-exact result as the
-oracle, for plain and multi-array (segmented large) sorts.  MGPUSORT_TILE,
-read at engine creation, forces one shape for every size."""
+"""Every built rank_scatter tile shape, with either in-wave ranking method,
+gives the same bit-exact result as the oracle, for plain and multi-array
+(segmented large) sorts.  Read at engine creation: MGPUSORT_TILE forces one
+shape for every size; MGPUSORT_RANK one ranking method (0: LDS match masks;
+1: one LDS atomic add per key, which the engine otherwise takes only after
+checking that the device orders same-word lanes)."""
 import os
 
 import numpy as np
@@ -12,20 +15,23 @@ from gpu_helpers import GpuSort, make_engine
 
 pytestmark = pytest.mark.gpu
 
-TILES = ["256x16", "512x16", "1024x16"]
+VARIANTS = [(t, r) for r in ("0", "1") for t in ("256x16", "512x16", "1024x16")]
+ENV = ("MGPUSORT_TILE", "MGPUSORT_RANK")
 
 
-@pytest.fixture(params=TILES)
+@pytest.fixture(params=VARIANTS, ids=[f"{t}_r{r}" for t, r in VARIANTS])
 def variant_engine(request):
-    saved = os.environ.get("MGPUSORT_TILE")
-    os.environ["MGPUSORT_TILE"] = request.param
+    tile, rank = request.param
+    saved = {k: os.environ.get(k) for k in ENV}
+    os.environ.update(MGPUSORT_TILE=tile, MGPUSORT_RANK=rank)
     try:
         eng = make_engine()
     finally:
-        if saved is None:
-            os.environ.pop("MGPUSORT_TILE", None)
-        else:
-            os.environ["MGPUSORT_TILE"] = saved
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     yield eng
     eng.close()
 
@@ -34,6 +40,9 @@ def variant_engine(request):
 @pytest.mark.parametrize("vc", [-1, 1, 3])
 def test_variant_matches_oracle(variant_engine, oracle, n, vc):
     keys = workloads.and3_keys(n, seed=n + vc)
+    if n > 5000:  # a run of equal digits and a stretch of one repeated key
+        keys[100:1100] &= 0xFFFF00FF
+        keys[2000:2600] = keys[2000]
     vals = [workloads.uniform_keys(n, seed=300 + j) for j in range(max(vc, 0))]
     g = GpuSort(keys, vals, value_count=vc).run(variant_engine)
     variant_engine.check_device()
