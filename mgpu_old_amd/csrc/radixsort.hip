@@ -78,6 +78,31 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s
     return before + inc - x;
 }
 
+// Adds 4 keys per lane (16-byte loads) to a per-wave LDS histogram of one
+// pass's digit.  Sorted and constant input put every lane on one counter, and
+// an LDS atomic serializes the lanes that hit the same word (2^28 sorted keys:
+// 2.6 ms instead of 0.3): when every active lane's 4 keys share one digit the
+// wave adds them with a single atomic.  (Merging equal digits within a lane
+// as well helps skewed input but costs uniform input more than it saves.)
+__device__ __forceinline__ void count4(uint32_t* h, uint32_t k0, uint32_t k1, uint32_t k2,
+                                       uint32_t k3, uint32_t shift, uint32_t mask) {
+    const uint32_t d0 = (k0 >> shift) & mask, d1 = (k1 >> shift) & mask;
+    const uint32_t d2 = (k2 >> shift) & mask, d3 = (k3 >> shift) & mask;
+    const uint32_t f = __builtin_amdgcn_readfirstlane(d0);
+    const uint64_t exec = __builtin_amdgcn_read_exec();
+    const bool one = d0 == f && d1 == f && d2 == f && d3 == f;
+    if (__builtin_amdgcn_ballot_w64(one) == exec) {
+        if (lanes_below(exec) == 0)
+            __hip_atomic_fetch_add(&h[f], 4u * (uint32_t)__builtin_popcountll(exec),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+    }
+    __hip_atomic_fetch_add(&h[d0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&h[d1], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&h[d2], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&h[d3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ---------------------------------------------------------------------------
 // digit_count: per-wave LDS histograms of all passes, merged into the global
 // histogram with one atomic per (pass, digit) per workgroup.  Also zeroes the
@@ -137,10 +162,10 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                 for (int u = 0; u < UNROLL; ++u) {
                     const uint32_t i = i0 + u * BLOCK;
                     if (i >= nvec) break;
-                    count_key(q[u].x);
-                    count_key(q[u].y);
-                    count_key(q[u].z);
-                    count_key(q[u].w);
+#pragma unroll
+                    for (int p = 0; p < MGPU_MAX_PASSES; ++p)
+                        if ((uint32_t)p < np)
+                            count4(h[p], q[u].x, q[u].y, q[u].z, q[u].w, a.shift[p], a.mask[p]);
                     if (check) {
                         descents += (bf(q[u].x) > bf(q[u].y)) + (bf(q[u].y) > bf(q[u].z)) +
                                     (bf(q[u].z) > bf(q[u].w));
@@ -170,7 +195,7 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
         if (i / RADIX >= np) break;
         uint32_t s = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) s += flat[w * MGPU_MAX_PASSES * RADIX + i];
+        for (int w = 0; w < W; ++w) s += s_hist[w][i / RADIX][i % RADIX];
         if (s) __hip_atomic_fetch_add(&a.hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -236,10 +261,10 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             if (i0 + u * BLOCK >= nvec) break;
-            count_key(q[u].x);
-            count_key(q[u].y);
-            count_key(q[u].z);
-            count_key(q[u].w);
+#pragma unroll
+            for (int p = 0; p < MGPU_MAX_PASSES; ++p)
+                if ((uint32_t)p < a.num_passes)
+                    count4(h[p], q[u].x, q[u].y, q[u].z, q[u].w, a.shift[p], a.mask[p]);
         }
     }
     for (uint32_t i = vbeg + 4 * nvec + tid; i < end; i += BLOCK) count_key(keys[i]);
@@ -249,7 +274,7 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
         if (i / RADIX >= np) break;
         uint32_t s = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) s += flat[w * MGPU_MAX_PASSES * RADIX + i];
+        for (int w = 0; w < W; ++w) s += s_hist[w][i / RADIX][i % RADIX];
         if (s) __hip_atomic_fetch_add(&hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
@@ -516,8 +541,20 @@ __device__ void rank_scatter_body(const Args& a) {
     for (int j = 0; j < IPT; ++j) {
         const uint32_t d = digit_of(j);
         if constexpr (RANK == 1) {
-            pos[j] = __hip_atomic_fetch_add(&s_count[wave][d], 1u, __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_WORKGROUP);
+            // A slot whose 64 keys share one digit (sorted or low-entropy
+            // input) takes its ranks with one add instead of 64 adds to one
+            // word, which the LDS would serialize.
+            const uint32_t f = __builtin_amdgcn_readfirstlane(d);
+            if (__builtin_amdgcn_ballot_w64(d == f) == ~0ull) {
+                uint32_t base = 0;
+                if (lane == 0)
+                    base = __hip_atomic_fetch_add(&s_count[wave][f], 64u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
+                pos[j] = __builtin_amdgcn_readfirstlane(base) + lane;
+            } else {
+                pos[j] = __hip_atomic_fetch_add(&s_count[wave][d], 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
         } else {
             // Each lane ORs its bit into the wave's mask for its digit and
             // reads the mask back; LDS operations of one wave complete in
