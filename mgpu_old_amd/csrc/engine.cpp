@@ -103,8 +103,11 @@ sortStatus_t mgpu_ensure_multi(SortLane& L, uint32_t arrays) {
         }
         L.multiArrays = cap;
     }
-    if (hipMemset(L.multi, 0, (size_t)L.multiArrays * MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t)) !=
-        hipSuccess)
+    // Stream-ordered: a plain hipMemset runs on the null stream, which a
+    // non-blocking lane stream does not wait for.
+    if (hipMemsetAsync(L.multi, 0,
+                       (size_t)L.multiArrays * MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
+                       L.stream) != hipSuccess)
         return SORT_STATUS_DEVICE_ERROR;
     L.multiDirty = false;
     return SORT_STATUS_SUCCESS;
@@ -116,7 +119,10 @@ sortStatus_t mgpu_lane_init(SortLane* l) {
         l->fixed = nullptr;
         return SORT_STATUS_DEVICE_ALLOC_FAILED;
     }
-    if (hipMemset(l->fixed, 0, words * sizeof(uint32_t)) != hipSuccess)
+    // On the lane's own stream, finished before the lane is handed out (the
+    // null stream would not order it before a non-blocking lane stream).
+    if (hipMemsetAsync(l->fixed, 0, words * sizeof(uint32_t), l->stream) != hipSuccess ||
+        hipStreamSynchronize(l->stream) != hipSuccess)
         return SORT_STATUS_DEVICE_ERROR;
     l->hist = l->fixed;
     l->digitStart = l->hist + MGPU_MAX_PASSES * MGPU_RADIX;
@@ -467,7 +473,8 @@ sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine) {
     if (hipMemcpy(&flag, engine->errorFlag, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
         return SORT_STATUS_DEVICE_ERROR;
     if (!flag) return SORT_STATUS_SUCCESS;
-    if (hipMemset(engine->errorFlag, 0, sizeof(uint32_t)) != hipSuccess)
+    if (hipMemsetAsync(engine->errorFlag, 0, sizeof(uint32_t), engine->lane.stream) != hipSuccess ||
+        hipStreamSynchronize(engine->lane.stream) != hipSuccess)
         return SORT_STATUS_DEVICE_ERROR;
     return SORT_STATUS_KERNEL_ERROR;
 }

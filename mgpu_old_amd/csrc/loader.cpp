@@ -105,7 +105,9 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
         e->errorFlag = nullptr;
         return SORT_STATUS_DEVICE_ALLOC_FAILED;
     }
-    if (hipMemset(e->errorFlag, 0, 16) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    if (hipMemsetAsync(e->errorFlag, 0, 16, e->lane.stream) != hipSuccess ||
+        hipStreamSynchronize(e->lane.stream) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
     return SORT_STATUS_SUCCESS;
 }
 

@@ -234,7 +234,11 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
              hipMemcpy(base + off + wideBytes + tbBytes, chunks.data(),
                        sizeof(uint32_t) * chunks.size(), hipMemcpyHostToDevice) != hipSuccess))
             return SORT_STATUS_DEVICE_ERROR;
-        if (hipMemset(base + payload, 0, ctrWords * sizeof(uint32_t)) != hipSuccess)
+        // Stream-ordered before the launches on the lane's stream (a plain
+        // hipMemset would run on the null stream, unordered with a
+        // non-blocking lane stream).
+        if (hipMemsetAsync(base + payload, 0, ctrWords * sizeof(uint32_t), lane.stream) !=
+            hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
     } else {
         // Counters live at the end of the arena and are only ever left zero
@@ -258,7 +262,11 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             arena->devBytes = cap;
             arena->hostBytes = payloadCap;
             arena->counterWords = words;
-            if (hipMemset(arena->dev, 0, cap) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+            // On the lane's stream, ahead of the descriptor copy below: a
+            // null-stream fill could land after that copy on a non-blocking
+            // lane stream and zero the descriptors the kernels then follow.
+            if (hipMemsetAsync(arena->dev, 0, cap, lane.stream) != hipSuccess)
+                return SORT_STATUS_DEVICE_ERROR;
         }
         base = static_cast<char*>(arena->dev);
         char* h = static_cast<char*>(arena->host);
@@ -496,9 +504,11 @@ sortStatus_t build_orientation(sortSegPlan_d* p, int o) {
     if (st) return st;
     SortLane& lane = e->lane;
     if (lane.histDirty) {  // no memset nodes inside the graph
-        if (hipMemset(lane.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t)) !=
-                hipSuccess ||
-            hipMemset(lane.tileCounters, 0, MGPU_LANE_COUNTERS * sizeof(uint32_t)) != hipSuccess)
+        if (hipMemsetAsync(lane.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
+                           lane.stream) != hipSuccess ||
+            hipMemsetAsync(lane.tileCounters, 0, MGPU_LANE_COUNTERS * sizeof(uint32_t),
+                           lane.stream) != hipSuccess ||
+            hipStreamSynchronize(lane.stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
         lane.histDirty = false;
     }

@@ -73,10 +73,6 @@ def test_served_requests_match_oracle_and_alone(engine, oracle):
     assert len(batches) > 1  # work really was split into several batches
 
 
-@pytest.mark.skip(reason="left the GPU with an illegal-address fault in one run, while a change "
-                         "giving each XCD its own lookback chains was in the tree; that change is "
-                         "taken out, but the fault's cause is not shown, so this case stays off "
-                         "the GPU until it is")
 def test_budget_admission_and_rejection(engine, oracle):
     import torch
     reqs, host, tensors = _make_requests(12, seed=21, lo_log2=14, hi_log2=18)
