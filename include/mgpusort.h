@@ -290,12 +290,12 @@ sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine);
 
 /* What the engine chose at creation, and the last device inconsistency code
  * (0 none; 1 a lookback wait gave up; 2 a tile counter left dirty; 4 a
- * scatter offset out of range) without clearing it.  No counterpart in the
- * reference. */
+ * scatter offset out of range; 5 a segment with no workgroup on its XCD)
+ * without clearing it.  No counterpart in the reference. */
 typedef struct {
     int device;
     int numCUs;
-    int reserved0;
+    int xcdSegments;   /* lookback chains per device-wide pass: 1, or one per XCD */
     int rankMethod;    /* 1: one LDS atomic add per key; 0: LDS match masks */
     int smallTileKeys; /* tile of passes below the switch */
     int largeTileKeys; /* tile of passes from the switch on */

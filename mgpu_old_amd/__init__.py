@@ -117,7 +117,7 @@ class EngineInfo(ctypes.Structure):
     _fields_ = [
         ("device", ctypes.c_int),
         ("numCUs", ctypes.c_int),
-        ("reserved0", ctypes.c_int),
+        ("xcdSegments", ctypes.c_int),
         ("rankMethod", ctypes.c_int),
         ("smallTileKeys", ctypes.c_int),
         ("largeTileKeys", ctypes.c_int),

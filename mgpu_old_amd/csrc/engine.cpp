@@ -114,7 +114,7 @@ sortStatus_t mgpu_ensure_multi(SortLane& L, uint32_t arrays) {
 }
 
 sortStatus_t mgpu_lane_init(SortLane* l) {
-    const size_t words = 2 * MGPU_MAX_PASSES * MGPU_RADIX + MGPU_LANE_COUNTERS;
+    const size_t words = 2 * kLaneHistWords + MGPU_LANE_COUNTERS + kLaneSegCountWords;
     if (hipMalloc(&l->fixed, words * sizeof(uint32_t)) != hipSuccess) {
         l->fixed = nullptr;
         return SORT_STATUS_DEVICE_ALLOC_FAILED;
@@ -125,8 +125,9 @@ sortStatus_t mgpu_lane_init(SortLane* l) {
         hipStreamSynchronize(l->stream) != hipSuccess)
         return SORT_STATUS_DEVICE_ERROR;
     l->hist = l->fixed;
-    l->digitStart = l->hist + MGPU_MAX_PASSES * MGPU_RADIX;
-    l->tileCounters = l->digitStart + MGPU_MAX_PASSES * MGPU_RADIX;
+    l->digitStart = l->hist + kLaneHistWords;
+    l->tileCounters = l->digitStart + kLaneHistWords;
+    l->segCount = l->tileCounters + MGPU_LANE_COUNTERS;
     l->histDirty = false;
     return SORT_STATUS_SUCCESS;
 }
@@ -209,10 +210,16 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
     sortStatus_t st = ensure_lookback(L, lbWords);
     if (st) return st;
+    // One segment per XCD (each with a lookback chain in its XCD's L2) for
+    // large sorts; else one chain across the device.  digit_count counts the
+    // segments of the input order, which only the first executed pass reads;
+    // each pass counts the next one's segments as it writes.
+    const uint32_t minTiles = e->xcdForce ? e->xcdSegs : kXsegMinTilesPerSeg * e->xcdSegs;
+    const uint32_t segs = e->xcdSegs > 1 && numTiles >= minTiles ? e->xcdSegs : 1u;
 
     if (L.histDirty) {
-        if (hipMemsetAsync(L.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
-                           L.stream) != hipSuccess ||
+        if (hipMemsetAsync(L.hist, 0, kLaneHistWords * sizeof(uint32_t), L.stream) !=
+                hipSuccess ||
             hipMemsetAsync(L.tileCounters, 0, MGPU_LANE_COUNTERS * sizeof(uint32_t), L.stream) !=
                 hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
@@ -244,17 +251,32 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         ca.check_shift = (uint32_t)firstBit;
         ca.check_mask = totalBits >= 32 ? 0xffffffffu : ((1u << totalBits) - 1u);
     }
-    uint32_t grid = (n + 8191) / 8192;
-    uint32_t maxGrid = (uint32_t)e->numCUs * 8;
-    if (grid > maxGrid) grid = maxGrid;
-    if (grid < 1) grid = 1;
-    ca.chunk = (uint32_t)(((uint64_t)n + grid - 1) / grid);
+    // Chunks of about 8192 keys, at most 8 workgroups per CU, cut per segment.
+    ca.num_segs = segs;
+    ca.tile = tile;
+    ca.num_tiles = numTiles;
+    const uint32_t maxGrid = (uint32_t)e->numCUs * 8;
+    uint64_t segKeys = 0;  // keys of the largest segment
+    for (uint32_t x = 0; x < segs; ++x) {
+        const uint64_t b = (uint64_t)mgpu_seg_first_tile(x, numTiles, segs) * tile;
+        const uint64_t en =
+            x + 1 == segs ? n : (uint64_t)mgpu_seg_first_tile(x + 1, numTiles, segs) * tile;
+        segKeys = std::max(segKeys, en - b);
+    }
+    uint32_t perSeg = (uint32_t)std::max<uint64_t>(1, (segKeys + 8191) / 8192);
+    perSeg = std::max(1u, std::min(perSeg, maxGrid / segs));
+    ca.chunk = (uint32_t)((segKeys + perSeg - 1) / perSeg);
     ca.chunk = (ca.chunk + 1023) & ~1023u;
-    grid = (n + ca.chunk - 1) / ca.chunk;
-    // Enough workgroups to zero the lookback words quickly as well.
+    ca.chunks_per_seg = (uint32_t)((segKeys + ca.chunk - 1) / ca.chunk);
+    uint32_t grid = ca.chunks_per_seg * segs;
+    // Enough workgroups to zero the lookback words quickly as well (the
+    // extra ones find their chunk empty).
     uint32_t clearGrid = (uint32_t)((ca.clear_words / 4 + 256 * 16 - 1) / (256 * 16));
     if (clearGrid > maxGrid) clearGrid = maxGrid;
-    if (clearGrid > grid) grid = clearGrid;
+    if (clearGrid > grid) {
+        ca.chunks_per_seg = (clearGrid + segs - 1) / segs;
+        grid = ca.chunks_per_seg * segs;
+    }
     L.histDirty = true;
     st = mgpu_launch(e, L.stream, e->fnCount, grid, 256, &ca);
     if (st) return st;
@@ -267,18 +289,25 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     for (int p = 0; p < numPasses; ++p) run[p] = true;
     bool sorted = false;
     if (earlyExit) {
-        std::vector<uint32_t> h((size_t)MGPU_MAX_PASSES * MGPU_RADIX + MGPU_LANE_COUNTERS);
-        if (hipMemcpyAsync(h.data(), L.hist, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
-            hipMemcpyAsync(h.data() + MGPU_MAX_PASSES * MGPU_RADIX, L.tileCounters,
-                           MGPU_LANE_COUNTERS * sizeof(uint32_t), hipMemcpyDeviceToHost,
+        const size_t hw = (size_t)segs * MGPU_MAX_PASSES * MGPU_RADIX;
+        std::vector<uint32_t> h(hw + MGPU_LANE_COUNTERS);
+        if (hipMemcpyAsync(h.data(), L.hist, hw * sizeof(uint32_t), hipMemcpyDeviceToHost,
                            L.stream) != hipSuccess ||
+            hipMemcpyAsync(h.data() + hw, L.tileCounters, MGPU_LANE_COUNTERS * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
             hipStreamSynchronize(L.stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
-        sorted = h[MGPU_MAX_PASSES * MGPU_RADIX + MGPU_UNSORTED_SLOT] == 0;
+        sorted = h[hw + MGPU_UNSORTED_SLOT] == 0;
         for (int p = 0; p < numPasses; ++p) {
-            const uint32_t* hp = h.data() + p * MGPU_RADIX;
-            run[p] = !sorted && *std::max_element(hp, hp + MGPU_RADIX) != n;
+            // The pass is the identity when one digit holds every key.
+            uint32_t most = 0;
+            for (uint32_t d = 0; d < MGPU_RADIX; ++d) {
+                uint32_t c = 0;
+                for (uint32_t x = 0; x < segs; ++x)
+                    c += h[(x * MGPU_MAX_PASSES + p) * MGPU_RADIX + d];
+                most = std::max(most, c);
+            }
+            run[p] = !sorted && most != n;
         }
     }
 
@@ -288,6 +317,10 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     sa.digit_start = L.digitStart;
     sa.tile_counters = L.tileCounters;
     sa.num_passes = (uint32_t)numPasses;
+    sa.num_segs = segs;
+    sa.seg_count = L.segCount;
+    sa.seg_count_words = segs > 1 ? (uint32_t)kLaneSegCountWords : 0u;
+    sa.pad_ = 0;
     st = mgpu_launch(e, L.stream, e->fnScan, 1, 256, &sa);
     if (st) return st;
     L.histDirty = false;
@@ -315,14 +348,33 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         }
         pa.digit_start = L.digitStart + p * MGPU_RADIX;
         pa.lookback = L.lookback + (uint64_t)p * numTiles * MGPU_RADIX;
-        pa.tile_counter = L.tileCounters + p;
+        pa.tile_counter = L.tileCounters + MGPU_TILE_COUNTER(p, 0);
+        pa.finished = L.tileCounters + MGPU_FINISHED_SLOT + p;
         pa.error_flag = e->errorFlag;
         pa.n = n;
         pa.num_tiles = numTiles;
         pa.shift = (uint32_t)shiftBits;
         pa.mask = (1u << widths[p]) - 1u;
         pa.num_values = (uint32_t)carried;
-        st = mgpu_launch(e, L.stream, var.pass[mode], numTiles, var.blockThreads, &pa);
+        pa.num_segs = segs;
+        if (segs > 1) {
+            pa.seg_count_in = done == 0 ? nullptr
+                                        : L.segCount + (size_t)p * MGPU_XSEG_COUNT_SEGS * MGPU_RADIX;
+            int nextRun = p + 1;
+            while (nextRun < numPasses && !run[nextRun]) ++nextRun;
+            if (nextRun < numPasses) {
+                pa.seg_count_out =
+                    L.segCount + (size_t)nextRun * MGPU_XSEG_COUNT_SEGS * MGPU_RADIX;
+                pa.next_shift = ca.shift[nextRun];
+                pa.next_mask = ca.mask[nextRun];
+            }
+            // Segment workgroups loop over their XCD's tiles: as many as fit
+            // at once (at least one per XCD).
+            const uint32_t g = std::max(segs, std::min(numTiles, var.xsegGrid));
+            st = mgpu_launch(e, L.stream, var.xseg[mode], g, var.blockThreads, &pa);
+        } else {
+            st = mgpu_launch(e, L.stream, var.pass[mode], numTiles, var.blockThreads, &pa);
+        }
         if (st) return st;
         ++done;
         if (executed) *executed = done;
@@ -484,6 +536,7 @@ sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info)
     std::memset(info, 0, sizeof(*info));
     info->device = engine->device;
     info->numCUs = engine->numCUs;
+    info->xcdSegments = (int)engine->xcdSegs;
     info->rankMethod = (int)engine->rankMethod;
     info->smallTileKeys = (int)engine->small.tile();
     info->largeTileKeys = (int)engine->large.tile();

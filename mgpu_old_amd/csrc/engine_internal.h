@@ -17,20 +17,30 @@
 struct PassVariant {
     uint32_t blockThreads = 0;
     uint32_t itemsPerThread = 0;
-    hipFunction_t pass[4] = {};   // mgpu_rank_scatter_b*_i*_v<mode>
-    hipFunction_t multi[4] = {};  // mgpu_multi_scatter_b*_i*_v<mode>
+    hipFunction_t pass[4] = {};   // mgpu_rank_scatter_b*_i*_r*_v<mode>: one device-wide chain
+    hipFunction_t xseg[4] = {};   // mgpu_rank_scatter_b*_i*_r*_x_v<mode>: a chain per XCD segment
+    hipFunction_t multi[4] = {};  // mgpu_multi_scatter_b*_i*_r*_v<mode>
+    uint32_t xsegGrid = 0;        // workgroups of an xseg launch: all that fit on the device
     uint32_t tile() const { return blockThreads * itemsPerThread; }
 };
+
+// Words of a lane's histogram (and of its digit starts): one set of pass
+// histograms per segment.
+constexpr size_t kLaneHistWords = (size_t)MGPU_MAX_SEGS * MGPU_MAX_PASSES * MGPU_RADIX;
+// Per-segment digit counts of each pass's input that the pass before it
+// fills in (segments per XCD).
+constexpr size_t kLaneSegCountWords = (size_t)MGPU_MAX_PASSES * MGPU_XSEG_COUNT_SEGS * MGPU_RADIX;
 
 // One in-order queue of sorts: a stream plus the device scratch of the
 // device-wide passes.  The engine owns one (its default lane, on the stream
 // chosen with sortSetStream); the sort server owns one per in-flight batch.
 struct SortLane {
     hipStream_t stream = nullptr;
-    uint32_t* fixed = nullptr;  // hist[4][256], digitStart[4][256], counters[8]
+    uint32_t* fixed = nullptr;  // hist, digitStart ([segments][4][256] each), counters, segCount
     uint32_t* hist = nullptr;
     uint32_t* digitStart = nullptr;
     uint32_t* tileCounters = nullptr;
+    uint32_t* segCount = nullptr;  // [MGPU_MAX_PASSES][MGPU_XSEG_COUNT_SEGS][MGPU_RADIX]
     bool histDirty = false;  // hist/tileCounters may hold a partial sort's counts
     uint32_t* lookback = nullptr;  // grow-only
     uint64_t lookbackWords = 0;
@@ -61,6 +71,12 @@ struct sortEngine_d {
     // tiles per CU, `small` below that (more, smaller tiles keep every CU busy).
     PassVariant small, large;
     uint32_t rankMethod = 0;  // 1: atomic-add ranking (checked at creation), 0: match masks
+    // Segments per XCD for single-array passes: the number of XCDs when the
+    // creation-time check passed (and MGPUSORT_XCD is not 0), else 1.
+    // xcdForce (MGPUSORT_XCD=1): use them at every size where each XCD gets
+    // a tile, not only from kXsegMinTilesPerSeg tiles per XCD.
+    uint32_t xcdSegs = 1;
+    bool xcdForce = false;
 
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;
@@ -99,6 +115,10 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
 // this many tiles per CU (2^22 keys on 256 CUs), else the small one (measured:
 // 1024 x 16 is ahead from 2^22 keys up, 512 x 16 below).
 constexpr uint32_t kLargeTilesPerCU = 1;
+// Segments per XCD pay a fixed cost per pass (each workgroup's counts for the
+// next pass are flushed once); they win from about this many tiles per XCD
+// (2^24 keys in 16384-key tiles on 8 XCDs).
+constexpr uint32_t kXsegMinTilesPerSeg = 128;
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys);
 
 // Grows the lane's lookback words to cover a device-wide sort of n keys in

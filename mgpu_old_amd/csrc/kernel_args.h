@@ -50,16 +50,33 @@ static inline MGPU_HOST_DEVICE uint32_t mgpu_rank_probe_digit(uint32_t wave, uin
     }
 }
 
-// digit_count: histograms of every pass's digit over all keys in one read.
+// A device-wide sort splits its tiles into num_segs contiguous segments, one
+// per XCD (num_segs = 1: one segment, one lookback chain across the device).
+// Segment x holds tiles [seg_first_tile(x), seg_first_tile(x + 1)) of
+// num_tiles; the workgroups of XCD x sort it with a lookback chain of its own,
+// which stays in that XCD's L2.
+#define MGPU_MAX_SEGS 16
+static inline MGPU_HOST_DEVICE uint32_t mgpu_seg_first_tile(uint32_t x, uint32_t num_tiles,
+                                                             uint32_t num_segs) {
+    return (uint32_t)(((uint64_t)x * num_tiles) / num_segs);
+}
+
+// digit_count: histograms of every pass's digit over all keys in one read,
+// one set per segment.  Workgroup b counts chunk (b % chunks_per_seg) of
+// segment (b / chunks_per_seg).
 struct mgpuCountArgs {
     const uint32_t* keys;
-    uint32_t* hist;          // [MGPU_MAX_PASSES][MGPU_RADIX], zero on entry
+    uint32_t* hist;          // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX], zero on entry
     uint32_t* clear;         // words zeroed on the way (the lookback words)
     uint64_t clear_words;    // multiple of 4
     uint32_t n;
-    uint32_t chunk;          // keys per workgroup, multiple of 4
+    uint32_t chunk;          // keys per workgroup, multiple of 1024
     uint32_t num_passes;
     uint32_t vec_ok;         // keys pointer is 16-byte aligned
+    uint32_t num_segs;
+    uint32_t chunks_per_seg;
+    uint32_t tile;           // keys per tile (segment bounds)
+    uint32_t num_tiles;
     uint32_t shift[MGPU_MAX_PASSES];
     uint32_t mask[MGPU_MAX_PASSES];
     // Sortedness check (early exit): counts adjacent pairs whose bit field
@@ -69,19 +86,26 @@ struct mgpuCountArgs {
     uint32_t check_mask;
 };
 
-// digit_scan: exclusive scan of each pass's histogram into digit starts; puts
-// the histogram and the tile counters back to zero for the next sort.
+// digit_scan: per pass, where each (segment, digit) run starts in the output
+// (digits in order, segments in order within a digit); puts the histograms
+// and the counters back to zero for the next sort.
 struct mgpuScanArgs {
-    uint32_t* hist;           // [MGPU_MAX_PASSES][MGPU_RADIX] -> zeroed
-    uint32_t* digit_start;    // [MGPU_MAX_PASSES][MGPU_RADIX]
-    uint32_t* tile_counters;  // [MGPU_LANE_COUNTERS] -> zeroed (tile counters, unsorted count)
+    uint32_t* hist;           // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX] -> zeroed
+    uint32_t* digit_start;    // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX]
+    uint32_t* tile_counters;  // [MGPU_LANE_COUNTERS] -> zeroed
     uint32_t num_passes;
+    uint32_t num_segs;
+    uint32_t* seg_count;      // per-segment counts the passes fill in -> zeroed
+    uint32_t seg_count_words;
+    uint32_t pad_;
 };
 
-// Words after digitStart in a lane's scratch: one tile counter per pass, then
-// the unsorted-pair count of the early-exit check.
-#define MGPU_LANE_COUNTERS 8
-#define MGPU_UNSORTED_SLOT MGPU_MAX_PASSES
+// Counter words of a lane: pass p's tile counter of segment x, the
+// workgroups of pass p that have finished, the early-exit descent count.
+#define MGPU_TILE_COUNTER(p, x) ((p) * MGPU_MAX_SEGS + (x))
+#define MGPU_FINISHED_SLOT (MGPU_MAX_PASSES * MGPU_MAX_SEGS)
+#define MGPU_UNSORTED_SLOT (MGPU_FINISHED_SLOT + MGPU_MAX_PASSES)
+#define MGPU_LANE_COUNTERS (MGPU_UNSORTED_SLOT + 4)
 
 // iota: out[i] = i for i < n (index output of a sort that needs no pass).
 struct mgpuIotaArgs {
@@ -169,20 +193,36 @@ struct mgpuMultiPassArgs {
 };
 
 // rank_scatter: one stable digit pass (rank in the tile, decoupled lookback
-// across tiles, LDS-staged scatter).
+// across tiles, LDS-staged scatter).  With num_segs > 1 the workgroups of XCD
+// x take the tiles of segment x (kernels built for it only).
 struct mgpuPassArgs {
     const uint32_t* keys_in;
     uint32_t* keys_out;
     const uint32_t* vals_in[MGPU_MAX_VALUES];
     uint32_t* vals_out[MGPU_MAX_VALUES];
-    const uint32_t* digit_start;  // [MGPU_RADIX] for this pass
+    const uint32_t* digit_start;  // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX] + this pass
     uint32_t* lookback;           // [num_tiles][MGPU_RADIX], zero on entry
-    uint32_t* tile_counter;       // zero on entry
+    uint32_t* tile_counter;       // this pass's [MGPU_MAX_SEGS] tile counters, zero on entry
+    uint32_t* finished;           // this pass's finished-workgroup count, zero on entry
     uint32_t* error_flag;         // set to 1 if a lookback wait gave up
     uint32_t n;
     uint32_t num_tiles;
     uint32_t shift;
     uint32_t mask;
     uint32_t num_values;          // MGPU_VALUES_MULTI only
-    uint32_t pad_;
+    uint32_t num_segs;
+    // Segments per XCD only.  A pass's segment starts are the device-wide
+    // digit starts plus the keys of the digit in earlier segments, counted
+    // over the keys in this pass's input order: from digit_count for the
+    // pass that reads the input (seg_count_in null: digit_start already holds
+    // per-segment starts), else from the pass before, which counts the next
+    // pass's digit of every key it writes by destination segment into
+    // seg_count_out ([MGPU_XSEG_COUNT_SEGS][MGPU_RADIX], zero on entry).
+    const uint32_t* seg_count_in;
+    uint32_t* seg_count_out;
+    uint32_t next_shift;
+    uint32_t next_mask;
 };
+
+// Segments per XCD with counting for the next pass: at most this many.
+#define MGPU_XSEG_COUNT_SEGS 8

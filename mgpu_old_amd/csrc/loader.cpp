@@ -3,6 +3,7 @@
 // over hipModuleLoad / hipModuleGetFunction instead of cuModuleLoad.
 // Names are kernelPath + "<object>.hsaco" (prefix concatenation, as the
 // reference builds kernelPath + "count.cubin").
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -37,8 +38,64 @@ sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, uin
         std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_r%u_v%d", threads, items,
                       rank, m);
         if (!st) st = get_fn(e, e->radixModule, name, &v->multi[m]);
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u_x_v%d", threads, items,
+                      rank, m);
+        if (!st) st = get_fn(e, e->radixModule, name, &v->xseg[m]);
     }
-    return st;
+    if (st) return st;
+    // Segment workgroups loop over tiles: launch as many as fit at once.
+    int perCU = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, v->xseg[MGPU_VALUES_SINGLE],
+                                                           (int)threads, 0) != hipSuccess ||
+        perCU < 1)
+        perCU = 1;
+    v->xsegGrid = (uint32_t)perCU * (uint32_t)e->numCUs;
+    return SORT_STATUS_SUCCESS;
+}
+
+// Segments per XCD need: the XCD a workgroup runs on (XCC_ID), a grid's
+// workgroups spread over every XCD, and lookback words that one CU writes with
+// nontemporal stores seen by another CU of the same XCD polling with
+// nontemporal loads.  mgpu_xcd_probe checks all three; returns the number of
+// XCDs to use (1: one device-wide chain).  MGPUSORT_XCD=0 turns them off,
+// MGPUSORT_XCD=1 uses them at every size (tests of small sorts).
+uint32_t xcd_segments(sortEngine_d* e) {
+    const char* env = std::getenv("MGPUSORT_XCD");
+    if (env && env[0] == '0' && env[1] == 0) return 1;
+    e->xcdForce = env && env[0] == '1' && env[1] == 0;
+    hipFunction_t f = nullptr;
+    if (get_fn(e, e->radixModule, "mgpu_xcd_probe", &f)) return 1;
+    constexpr uint32_t kGrid = 8 * MGPU_MAX_SEGS, kRounds = 64;
+    const size_t outWords = kGrid + 2 * MGPU_MAX_SEGS, areaWords = MGPU_MAX_SEGS * 128;
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, (outWords + areaWords) * sizeof(uint32_t)) != hipSuccess) return 1;
+    std::vector<uint32_t> h(outWords);
+    uint32_t* out = d;
+    uint32_t* area = d + outWords;
+    uint32_t rounds = kRounds;
+    void* params[] = {&out, &area, &rounds};
+    bool ran = hipMemsetAsync(d, 0, (outWords + areaWords) * sizeof(uint32_t), e->lane.stream) ==
+               hipSuccess;
+    ran = ran && hipModuleLaunchKernel(f, kGrid, 1, 1, 64, 1, 1, 0, e->lane.stream, params,
+                                       nullptr) == hipSuccess;
+    ran = ran && hipMemcpyAsync(h.data(), d, outWords * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                e->lane.stream) == hipSuccess;
+    ran = ran && hipStreamSynchronize(e->lane.stream) == hipSuccess;
+    (void)hipFree(d);
+    if (!ran) return 1;
+    uint32_t xcds = 0;
+    uint32_t per[MGPU_MAX_SEGS] = {};
+    for (uint32_t b = 0; b < kGrid; ++b) {
+        if (h[b] >= MGPU_MAX_SEGS) return 1;
+        per[h[b]]++;
+        xcds = std::max(xcds, h[b] + 1);
+    }
+    if (xcds < 2 || xcds > MGPU_XSEG_COUNT_SEGS) return 1;
+    for (uint32_t x = 0; x < xcds; ++x) {
+        if (per[x] < 2) return 1;  // an XCD without workgroups, or without a partner
+        if (h[kGrid + 2 * x] != kRounds || h[kGrid + 2 * x + 1] != 0) return 1;
+    }
+    return xcds;
 }
 
 // Whether the device serves the lanes of one LDS atomic add that hit the same
@@ -98,6 +155,7 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     const char* r = std::getenv("MGPUSORT_RANK");
     if (r && (*r == '0' || *r == '1') && r[1] == 0) e->rankMethod = (uint32_t)(*r - '0');
     else e->rankMethod = lds_add_in_lane_order(e) ? 1u : 0u;
+    e->xcdSegs = xcd_segments(e);
     st = load_variant(e, smallShape[0], smallShape[1], e->rankMethod, &e->small);
     if (!st) st = load_variant(e, largeShape[0], largeShape[1], e->rankMethod, &e->large);
     if (st) return st;

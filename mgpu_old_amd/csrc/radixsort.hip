@@ -103,6 +103,20 @@ __device__ __forceinline__ void count4(uint32_t* h, uint32_t k0, uint32_t k1, ui
     __hip_atomic_fetch_add(&h[d3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Adds 1 per active lane to LDS counter h[i], without branches.  The lanes of
+// one LDS atomic that hit the same word are serialized, so when every active
+// lane has the same i the first active lane adds their number and the others
+// add 0 to a word of their own in spare[64].
+__device__ __forceinline__ void count1(uint32_t* h, uint32_t* spare, uint32_t i, uint32_t lane) {
+    const uint32_t f = __builtin_amdgcn_readfirstlane(i);
+    const uint64_t exec = __builtin_amdgcn_read_exec();
+    const bool one = __builtin_amdgcn_ballot_w64(i == f) == exec;
+    const bool first = lanes_below(exec) == 0;
+    uint32_t* const w = one && !first ? &spare[lane] : &h[i];
+    const uint32_t v = one ? (first ? (uint32_t)__builtin_popcountll(exec) : 0u) : 1u;
+    __hip_atomic_fetch_add(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ---------------------------------------------------------------------------
 // digit_count: per-wave LDS histograms of all passes, merged into the global
 // histogram with one atomic per (pass, digit) per workgroup.  Also zeroes the
@@ -135,8 +149,15 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
     };
 
-    const uint32_t begin = blockIdx.x * a.chunk;
-    const uint32_t end = min(a.n, begin + a.chunk);
+    // Chunk of this workgroup inside its segment; the segment's histograms.
+    const uint32_t seg = blockIdx.x / a.chunks_per_seg;
+    const uint32_t seg_begin =
+        seg == 0 ? 0u : mgpu_seg_first_tile(seg, a.num_tiles, a.num_segs) * a.tile;
+    const uint32_t seg_end =
+        seg + 1 >= a.num_segs ? a.n : mgpu_seg_first_tile(seg + 1, a.num_tiles, a.num_segs) * a.tile;
+    const uint32_t begin = min(seg_end, seg_begin + (blockIdx.x % a.chunks_per_seg) * a.chunk);
+    const uint32_t end = min(seg_end, begin + a.chunk);
+    uint32_t* const hist = a.hist + seg * (MGPU_MAX_PASSES * RADIX);
     // Early-exit check: descents of the sort's bit field between neighbours
     // (each key against the one after it, across chunk ends too).
     const bool check = a.unsorted != nullptr;
@@ -196,22 +217,33 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
         uint32_t s = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) s += s_hist[w][i / RADIX][i % RADIX];
-        if (s) __hip_atomic_fetch_add(&a.hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s) __hip_atomic_fetch_add(&hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
 // ---------------------------------------------------------------------------
-// digit_scan: one workgroup of 256 threads.
+// digit_scan: one workgroup of 256 threads (thread d: digit d).  A pass's
+// output holds the digits in order and, within a digit, the segments in
+// order: run (x, d) starts at the keys of smaller digits plus the keys of
+// digit d in segments before x.
 __device__ void digit_scan_body(const mgpuScanArgs& a) {
     __shared__ uint32_t s_tmp[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (uint32_t p = 0; p < a.num_passes; ++p) {
-        uint32_t c = a.hist[p * RADIX + tid];
-        uint32_t e = block_exclusive_scan<256>(c, s_tmp, lane, wave);
-        a.digit_start[p * RADIX + tid] = e;
-        a.hist[p * RADIX + tid] = 0;
+        uint32_t total = 0;
+        for (uint32_t x = 0; x < a.num_segs; ++x)
+            total += a.hist[(x * MGPU_MAX_PASSES + p) * RADIX + tid];
+        uint32_t run = block_exclusive_scan<256>(total, s_tmp, lane, wave);
+        for (uint32_t x = 0; x < a.num_segs; ++x) {
+            const uint32_t i = (x * MGPU_MAX_PASSES + p) * RADIX + tid;
+            const uint32_t c = a.hist[i];
+            a.digit_start[i] = run;
+            run += c;
+            a.hist[i] = 0;
+        }
     }
-    if (tid < MGPU_LANE_COUNTERS) a.tile_counters[tid] = 0;
+    for (uint32_t i = tid; i < MGPU_LANE_COUNTERS; i += 256) a.tile_counters[i] = 0;
+    for (uint32_t i = tid; i < a.seg_count_words; i += 256) a.seg_count[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -310,20 +342,38 @@ __device__ void multi_scan_body(const mgpuMultiScanArgs& a) {
 #define MGPU_LOOKBACK_WINDOW 16
 #endif
 
+// LOCAL chains (one XCD's segment, read and written only by that XCD's
+// workgroups) use nontemporal loads and stores, which reach the XCD's L2 but
+// do not stay in a CU's L1, so a poll sees another CU's store at L2 latency.
+// The engine checks that behaviour (mgpu_xcd_probe) before it picks LOCAL
+// kernels.  Other chains use agent-scope atomics, coherent across XCDs.
+template <bool LOCAL>
 __device__ __forceinline__ uint32_t lookback_word(const uint32_t* lb, int32_t t, uint32_t d) {
-    return t >= 0 ? ld_agent(&lb[(uint32_t)t * RADIX + d]) : MGPU_LB_PREFIX;
+    if (t < 0) return MGPU_LB_PREFIX;
+    const uint32_t* p = &lb[(uint32_t)t * RADIX + d];
+    if constexpr (LOCAL) return __builtin_nontemporal_load(p);
+    else return ld_agent(p);
 }
 
-template <uint32_t K>
+template <bool LOCAL>
+__device__ __forceinline__ void lookback_publish(uint32_t* p, uint32_t v) {
+    if constexpr (LOCAL) __builtin_nontemporal_store(v, p);
+    else st_agent(p, v);
+}
+
+template <uint32_t K, bool LOCAL>
 __device__ __forceinline__ void lookback_read(const uint32_t* lb, int32_t t0, uint32_t d,
                                               uint32_t k, uint32_t (&w)[K]) {
+    // Plain (nontemporal) loads: keep the compiler from reusing an earlier
+    // round's values.
+    if constexpr (LOCAL) asm volatile("" ::: "memory");
 #pragma unroll
-    for (uint32_t q = 0; q < K; ++q) w[q] = lookback_word(lb, t0 - (int32_t)(k * K + q), d);
+    for (uint32_t q = 0; q < K; ++q) w[q] = lookback_word<LOCAL>(lb, t0 - (int32_t)(k * K + q), d);
 }
 
 // Returns the exclusive prefix of the tile for digit d.  `w` holds the first
 // round's words, read by the caller with lookback_read(lb, tile - 1, ...).
-template <uint32_t G, uint32_t K>
+template <uint32_t G, uint32_t K, bool LOCAL>
 __device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t tile, uint32_t d,
                                                     uint32_t k, uint32_t lane,
                                                     uint32_t* error_flag, uint32_t (&w)[K]) {
@@ -354,7 +404,7 @@ __device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t
                 return prefix;
             }
             __builtin_amdgcn_s_sleep(1);
-            lookback_read<K>(lb, t0, d, k, w);
+            lookback_read<K, LOCAL>(lb, t0, d, k, w);
             continue;
         }
         uint32_t v = ((upto >> k) & 1u) ? sum : 0u;
@@ -363,7 +413,7 @@ __device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t
         prefix += v;
         if (pref) return prefix;
         t0 -= (int32_t)(G * K);
-        lookback_read<K>(lb, t0, d, k, w);
+        lookback_read<K, LOCAL>(lb, t0, d, k, w);
     }
 }
 
@@ -373,9 +423,10 @@ struct TileJob {
     uint32_t* keys_out;
     const mgpuSegDesc* sd;  // MULTI: the array's descriptor
     const uint32_t* digit_start;
-    uint32_t* lookback;     // lookback words from the array's first tile on
+    uint32_t* lookback;     // lookback words from the chain's first tile on
     uint32_t n;             // keys of the array
-    uint32_t tile;          // tile index within the array
+    uint32_t tile;          // tile index within the chain
+    uint32_t first;         // tiles of the array before the chain's first tile
 };
 
 // Global tile t of a pass: which array (MULTI) and where its data lives.
@@ -406,6 +457,7 @@ __device__ __forceinline__ TileJob make_job(const Args& a, uint32_t gtile, uint3
         j.keys_out = j.sd->keys[a.in_side ^ 1];
         j.n = j.sd->n;
         j.tile = gtile - tb;
+        j.first = 0;
         j.digit_start = a.digit_start + (seg * MGPU_MAX_PASSES + a.pass) * RADIX;
         j.lookback = a.lookback + (size_t)tb * RADIX;
     } else {
@@ -415,6 +467,7 @@ __device__ __forceinline__ TileJob make_job(const Args& a, uint32_t gtile, uint3
         j.keys_out = a.keys_out;
         j.n = a.n;
         j.tile = gtile;
+        j.first = 0;
         j.digit_start = a.digit_start;
         j.lookback = a.lookback;
     }
@@ -463,8 +516,11 @@ __device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t n, ui
 // instruction that hit the same count are served in ascending lane order,
 // which the engine checks at creation (mgpu_rank_probe) before it picks
 // these kernels.
-template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, typename Args>
+// XSEG (single array only): the workgroup takes tiles of its XCD's segment
+// until the segment is done; the segment has a lookback chain of its own.
+template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool XSEG, typename Args>
 __device__ void rank_scatter_body(const Args& a) {
+    static_assert(!(MULTI && XSEG), "segments per XCD are for single arrays");
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
     constexpr uint32_t TILE = BLOCK * IPT;
@@ -487,193 +543,303 @@ __device__ void rank_scatter_body(const Args& a) {
     __shared__ uint32_t s_tile[2];
     __shared__ uint32_t s_lstart[RADIX];
     __shared__ uint32_t s_tcount[RADIX];
+    // XSEG: this segment's digit starts, and the next pass's digit counts of
+    // the keys this workgroup writes, by destination segment.
+    constexpr uint32_t NSEG = MGPU_XSEG_COUNT_SEGS;
+    __shared__ uint32_t s_dstart[XSEG ? RADIX : 1];
+    __shared__ uint32_t s_next[XSEG ? NSEG * RADIX : 1];
+    __shared__ uint32_t s_spare[XSEG ? 64 : 1];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
     const uint32_t shift = a.shift, mask = a.mask;
 
-    // The tile comes from the counter in start order, so a tile only ever
-    // waits on tiles that are already running.
-    if (tid == 0) {
-        const uint32_t t = __hip_atomic_fetch_add(a.tile_counter, 1u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        s_tile[0] = t;
-        s_tile[1] = t < a.num_tiles ? find_array<MULTI>(a, t) : 0u;
+    // The segment of this workgroup's XCD (XSEG), else the whole array.
+    uint32_t seg = 0, seg_first = 0, seg_tiles = a.num_tiles;
+    if constexpr (XSEG) {
+        uint32_t xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        seg = xcc & 0xFu;
+        if (seg < a.num_segs) {
+            seg_first = mgpu_seg_first_tile(seg, a.num_tiles, a.num_segs);
+            seg_tiles = mgpu_seg_first_tile(seg + 1, a.num_tiles, a.num_segs) - seg_first;
+        } else {
+            seg_tiles = 0;  // no segment for this XCD: nothing to take
+        }
     }
     for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
         (&s_count[0][0])[i] = 0;
         if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
     }
-    __syncthreads();
-    // Tile and array indices are the same for every lane: keep them (and all
-    // that is derived from them) in scalar registers.
-    const uint32_t gtile = __builtin_amdgcn_readfirstlane(s_tile[0]);
-    if (gtile >= a.num_tiles) {  // only a counter left dirty gets here
-        if (tid == 0) st_agent(a.error_flag, 2u);
-        return;
-    }
-    const TileJob job = make_job<MULTI>(a, gtile, __builtin_amdgcn_readfirstlane(s_tile[1]));
-    const uint32_t n = job.n, tile = job.tile;
-    const uint32_t base = tile * TILE;
-    const uint32_t wave_base = base + wave * WAVE_TILE;
-    const uint32_t tile_valid = min(TILE, n - base);
-    const bool full = tile_valid == TILE;
-
-    // Keys and the single value array, striped, requested together: ranking
-    // starts as soon as each key slot is in while the values still load.
-    uint32_t key[IPT], val[IPT];
-    load_striped<IPT>(job.keys_in, n, wave_base, lane, key);
-    if constexpr (VMODE == MGPU_VALUES_SINGLE)
-        load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), n, wave_base, lane, val);
-
-    // Digit of slot j; keys past n get the top digit so that they rank after
-    // every real key of the tile (they are last in input order).
-    const int32_t rem = (int32_t)(n - (wave_base + lane));  // slots j*64 < rem are real
-    auto digit_of = [&](int j) -> uint32_t {
-        if (full) return (key[j] >> shift) & mask;
-        return j * 64 < rem ? ((key[j] >> shift) & mask) : (RADIX - 1);
-    };
-
-    // Rank within the wave: pos[j] = earlier same-digit keys of this wave.
-    uint32_t pos[IPT];
+    // XSEG: where the next pass's segments start (uniform; past num_segs:
+    // never), whether this pass counts for it, and this segment's starts.
+    uint32_t next_bound[NSEG];
+    bool count_next = false;
+    if constexpr (XSEG) {
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const uint32_t d = digit_of(j);
-        if constexpr (RANK == 1) {
-            // A slot whose 64 keys share one digit (sorted or low-entropy
-            // input) takes its ranks with one add instead of 64 adds to one
-            // word, which the LDS would serialize.
-            const uint32_t f = __builtin_amdgcn_readfirstlane(d);
-            if (__builtin_amdgcn_ballot_w64(d == f) == ~0ull) {
-                uint32_t base = 0;
-                if (lane == 0)
-                    base = __hip_atomic_fetch_add(&s_count[wave][f], 64u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-                pos[j] = __builtin_amdgcn_readfirstlane(base) + lane;
+        for (uint32_t x = 0; x < NSEG; ++x)
+            next_bound[x] = x < a.num_segs ? mgpu_seg_first_tile(x, a.num_tiles, a.num_segs) * TILE
+                                           : 0xFFFFFFFFu;
+        count_next = a.seg_count_out != nullptr;
+        if (count_next)
+            for (uint32_t i = tid; i < NSEG * RADIX; i += BLOCK) s_next[i] = 0;
+        if (tid < RADIX && seg < a.num_segs) {
+            uint32_t st;
+            if (a.seg_count_in) {
+                st = a.digit_start[tid];
+                for (uint32_t x = 0; x < seg; ++x) st += a.seg_count_in[x * RADIX + tid];
             } else {
-                pos[j] = __hip_atomic_fetch_add(&s_count[wave][d], 1u, __ATOMIC_RELAXED,
-                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                st = a.digit_start[seg * (MGPU_MAX_PASSES * RADIX) + tid];
             }
-        } else {
-            // Each lane ORs its bit into the wave's mask for its digit and
-            // reads the mask back; LDS operations of one wave complete in
-            // order, so the mask holds exactly the lanes of this slot with
-            // that digit.
-            __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_WORKGROUP);
-            const uint64_t m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-            const uint32_t below = lanes_below(m);
-            const uint32_t old = s_count[wave][d];
-            pos[j] = old + below;
-            if (below == 0) {  // lowest lane of the group advances the count
-                s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
-                __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_WORKGROUP);
-            }
+            s_dstart[tid] = st;
         }
+    } else {
+        (void)next_bound;
     }
-    __syncthreads();
 
-    // Per digit (thread d): wave-exclusive offsets and the tile total,
-    // published at once for the tiles after this one; then the tile-local
-    // digit starts (block scan) are added to the wave offsets.
-    uint32_t total = 0, tile_count = 0;
-    if (tid < RADIX) {
-        uint32_t sum = 0;
-#pragma unroll
-        for (int w = 0; w < W; ++w) {
-            const uint32_t c = s_count[w][tid];
-            s_count[w][tid] = sum;
-            sum += c;
+    while (true) {
+        // The tile comes from the counter in start order (and is sorted right
+        // away), so a tile only ever waits on tiles that are already running.
+        if (tid == 0) {
+            uint32_t t = seg_tiles;  // a workgroup of an XCD without a segment takes none
+            if (seg_tiles > 0)
+                t = __hip_atomic_fetch_add(a.tile_counter + seg, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            s_tile[0] = t;
+            s_tile[1] = !XSEG && t < a.num_tiles ? find_array<MULTI>(a, t) : 0u;
         }
-        total = sum;
-        tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
-        st_agent(&job.lookback[tile * RADIX + tid],
-                 (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
-    }
-    const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
-    if (tid < RADIX) {
-        s_lstart[tid] = local_start;
-        s_tcount[tid] = tile_count;
-#pragma unroll
-        for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
-    }
-    __syncthreads();
-
-    // BLOCK / RADIX threads per digit walk back over the predecessors.  The
-    // first round's words are requested before the staging below so that
-    // their latency overlaps it (starting earlier only waits longer for the
-    // predecessors' own totals).
-    constexpr uint32_t G = BLOCK / RADIX;
-    constexpr uint32_t K = MGPU_LOOKBACK_WINDOW / G > 0 ? MGPU_LOOKBACK_WINDOW / G : 1;
-    const uint32_t lb_digit = tid / G, lb_k = tid % G;
-    uint32_t lb_words[K];
-    lookback_read<K>(job.lookback, (int32_t)tile - 1, lb_digit, lb_k, lb_words);
-
-    // Tile-local position of every key; stage the keys in digit order (and,
-    // with one value array or generated indices, the values beside them, so
-    // that both leave after a single barrier).
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        pos[j] += s_count[wave][digit_of(j)];
-        s_stage[pos[j]] = key[j];
-        if constexpr (JOINT) {
-            if constexpr (VMODE == MGPU_VALUES_INDEX) s_vstage[pos[j]] = wave_base + lane + j * 64;
-            else s_vstage[pos[j]] = val[j];
+        __syncthreads();
+        // Tile and array indices are the same for every lane: keep them (and all
+        // that is derived from them) in scalar registers.
+        const uint32_t gtile = __builtin_amdgcn_readfirstlane(s_tile[0]);
+        if (gtile >= seg_tiles) {
+            // One tile per workgroup: only a counter left dirty gets here.  A
+            // segment's workgroup: the segment is done.
+            if (!XSEG && tid == 0) st_agent(a.error_flag, 2u);
+            break;
         }
-    }
-
-    {
-        const uint32_t prefix = lookback_window<G, K>(job.lookback, tile, lb_digit, lb_k, lane,
-                                                      a.error_flag, lb_words);
-        if (lb_k == 0) {
-            if (tile > 0)
-                st_agent(&job.lookback[tile * RADIX + lb_digit],
-                         MGPU_LB_PREFIX | (prefix + s_tcount[lb_digit]));
-            s_global[lb_digit] = job.digit_start[lb_digit] + prefix - s_lstart[lb_digit];
+        TileJob job = make_job<MULTI>(a, gtile, __builtin_amdgcn_readfirstlane(s_tile[1]));
+        if constexpr (XSEG) {
+            job.first = seg_first;
+            job.lookback += (size_t)seg_first * RADIX;
         }
-    }
-    __syncthreads();
+        const uint32_t n = job.n, tile = job.tile;
+        const uint32_t base = (job.first + tile) * TILE;
+        const uint32_t wave_base = base + wave * WAVE_TILE;
+        const uint32_t tile_valid = min(TILE, n - base);
+        const bool full = tile_valid == TILE;
 
-    // Write out in tile order: consecutive threads hold consecutive positions,
-    // so each digit's run of the tile leaves as one stretch.
-    constexpr bool KEEP_DEST = VMODE == MGPU_VALUES_MULTI;
-    uint32_t dest[KEEP_DEST ? IPT : 1];
-    const int32_t vrem = (int32_t)(tile_valid - tid);  // positions i*BLOCK < vrem are real
-    uint32_t* const vout0 = JOINT ? job_vals_out<MULTI>(a, job, 0) : nullptr;
+        // Keys and the single value array, striped, requested together: ranking
+        // starts as soon as each key slot is in while the values still load.
+        uint32_t key[IPT], val[IPT];
+        load_striped<IPT>(job.keys_in, n, wave_base, lane, key);
+        if constexpr (VMODE == MGPU_VALUES_SINGLE)
+            load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), n, wave_base, lane, val);
+
+        // Digit of slot j; keys past n get the top digit so that they rank after
+        // every real key of the tile (they are last in input order).
+        const int32_t rem = (int32_t)(n - (wave_base + lane));  // slots j*64 < rem are real
+        auto digit_of = [&](int j) -> uint32_t {
+            if (full) return (key[j] >> shift) & mask;
+            return j * 64 < rem ? ((key[j] >> shift) & mask) : (RADIX - 1);
+        };
+
+        // Rank within the wave: pos[j] = earlier same-digit keys of this wave.
+        uint32_t pos[IPT];
 #pragma unroll
-    for (int i = 0; i < IPT; ++i) {
-        const uint32_t p = i * BLOCK + tid;
-        uint32_t dst = n;
-        if (full || i * BLOCK < vrem) {
-            const uint32_t k = s_stage[p];
-            dst = s_global[(k >> shift) & mask] + p;
-            // Offsets outside [0, n) can only come from a broken count or
-            // lookback: flag it instead of writing out of bounds.
-            if (dst < n) {
-                job.keys_out[dst] = k;
-                if constexpr (JOINT) vout0[dst] = s_vstage[p];
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t d = digit_of(j);
+            if constexpr (RANK == 1) {
+                // A slot whose 64 keys share one digit (sorted or low-entropy
+                // input) takes its ranks with one add instead of 64 adds to one
+                // word, which the LDS would serialize.
+                const uint32_t f = __builtin_amdgcn_readfirstlane(d);
+                if (__builtin_amdgcn_ballot_w64(d == f) == ~0ull) {
+                    uint32_t base = 0;
+                    if (lane == 0)
+                        base = __hip_atomic_fetch_add(&s_count[wave][f], 64u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                    pos[j] = __builtin_amdgcn_readfirstlane(base) + lane;
+                } else {
+                    pos[j] = __hip_atomic_fetch_add(&s_count[wave][d], 1u, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             } else {
-                st_agent(a.error_flag, 4u);
+                // Each lane ORs its bit into the wave's mask for its digit and
+                // reads the mask back; LDS operations of one wave complete in
+                // order, so the mask holds exactly the lanes of this slot with
+                // that digit.
+                __hip_atomic_fetch_or(&s_match[wave][d], 1ull << lane, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint64_t m = __hip_atomic_load(&s_match[wave][d], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+                const uint32_t below = lanes_below(m);
+                const uint32_t old = s_count[wave][d];
+                pos[j] = old + below;
+                if (below == 0) {  // lowest lane of the group advances the count
+                    s_count[wave][d] = old + (uint32_t)__builtin_popcountll(m);
+                    __hip_atomic_store(&s_match[wave][d], 0ull, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
             }
         }
-        if constexpr (KEEP_DEST) dest[i] = dst;
-    }
+        __syncthreads();
 
-    // Two or more value arrays: one after the other through the key stage.
-    if constexpr (VMODE == MGPU_VALUES_MULTI) {
-        for (uint32_t v = 0; v < a.num_values; ++v) {
-            load_striped<IPT>(job_vals_in<MULTI>(a, job, v), n, wave_base, lane, val);
-            __syncthreads();
+        // Per digit (thread d): wave-exclusive offsets and the tile total,
+        // published at once for the tiles after this one; then the tile-local
+        // digit starts (block scan) are added to the wave offsets.
+        uint32_t total = 0, tile_count = 0;
+        if (tid < RADIX) {
+            uint32_t sum = 0;
 #pragma unroll
-            for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
-            __syncthreads();
-            uint32_t* dstv = job_vals_out<MULTI>(a, job, v);
+            for (int w = 0; w < W; ++w) {
+                const uint32_t c = s_count[w][tid];
+                s_count[w][tid] = sum;
+                sum += c;
+            }
+            total = sum;
+            tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
+            lookback_publish<XSEG>(&job.lookback[tile * RADIX + tid],
+                                   (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
+        }
+        const uint32_t local_start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
+        if (tid < RADIX) {
+            s_lstart[tid] = local_start;
+            s_tcount[tid] = tile_count;
 #pragma unroll
-            for (int i = 0; i < IPT; ++i)
-                if ((full || i * BLOCK < vrem) && dest[i] < n) dstv[dest[i]] = s_stage[i * BLOCK + tid];
+            for (int w = 0; w < W; ++w) s_count[w][tid] += local_start;
+        }
+        __syncthreads();
+
+        // BLOCK / RADIX threads per digit walk back over the predecessors.  The
+        // first round's words are requested before the staging below so that
+        // their latency overlaps it (starting earlier only waits longer for the
+        // predecessors' own totals).
+        constexpr uint32_t G = BLOCK / RADIX;
+        constexpr uint32_t K = MGPU_LOOKBACK_WINDOW / G > 0 ? MGPU_LOOKBACK_WINDOW / G : 1;
+        const uint32_t lb_digit = tid / G, lb_k = tid % G;
+        uint32_t lb_words[K];
+        lookback_read<K, XSEG>(job.lookback, (int32_t)tile - 1, lb_digit, lb_k, lb_words);
+
+        // Tile-local position of every key; stage the keys in digit order (and,
+        // with one value array or generated indices, the values beside them, so
+        // that both leave after a single barrier).
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            pos[j] += s_count[wave][digit_of(j)];
+            s_stage[pos[j]] = key[j];
+            if constexpr (JOINT) {
+                if constexpr (VMODE == MGPU_VALUES_INDEX) s_vstage[pos[j]] = wave_base + lane + j * 64;
+                else s_vstage[pos[j]] = val[j];
+            }
+        }
+
+        {
+            const uint32_t prefix = lookback_window<G, K, XSEG>(job.lookback, tile, lb_digit, lb_k,
+                                                                lane, a.error_flag, lb_words);
+            if (lb_k == 0) {
+                if (tile > 0)
+                    lookback_publish<XSEG>(&job.lookback[tile * RADIX + lb_digit],
+                                           MGPU_LB_PREFIX | (prefix + s_tcount[lb_digit]));
+                const uint32_t dstart = XSEG ? s_dstart[lb_digit] : job.digit_start[lb_digit];
+                s_global[lb_digit] = dstart + prefix - s_lstart[lb_digit];
+            }
+        }
+        __syncthreads();
+
+        // Write out in tile order: consecutive threads hold consecutive
+        // positions, so each digit's run of the tile leaves as one stretch.
+        // The LDS reads of every slot are issued before the first store, so
+        // that their latencies overlap instead of adding up slot by slot
+        // (slots past the tile's keys read LDS words that are not used).
+        const int32_t vrem = (int32_t)(tile_valid - tid);  // positions i*BLOCK < vrem are real
+        uint32_t* const vout0 = JOINT ? job_vals_out<MULTI>(a, job, 0) : nullptr;
+        uint32_t kout[IPT], dest[IPT], vout[JOINT ? IPT : 1];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) kout[i] = s_stage[i * BLOCK + tid];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            dest[i] = s_global[(kout[i] >> shift) & mask] + i * BLOCK + tid;
+            if constexpr (JOINT) vout[i] = s_vstage[i * BLOCK + tid];
+        }
+        // Offsets outside [0, n) can only come from a broken count or
+        // lookback: they are flagged instead of written (dest = n).
+        bool bad = false;
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) {
+            const bool real = full || i * BLOCK < vrem;
+            const bool ok = real && dest[i] < n;
+            bad = bad || (real && !ok);
+            if (!ok) dest[i] = n;
+            if (ok) {
+                job.keys_out[dest[i]] = kout[i];
+                if constexpr (JOINT) vout0[dest[i]] = vout[i];
+            }
+        }
+        if (bad) st_agent(a.error_flag, 4u);
+        // Segments per XCD: the next pass's digit of every key written, by
+        // the segment of the next pass that its position falls in.
+        if constexpr (XSEG) {
+            if (count_next) {
+#pragma unroll
+                for (int i = 0; i < IPT; ++i) {
+                    uint32_t xo = 0;
+#pragma unroll
+                    for (uint32_t x = 1; x < NSEG; ++x) xo += dest[i] >= next_bound[x];
+                    if (dest[i] < n)
+                        count1(s_next, s_spare, xo * RADIX + ((kout[i] >> a.next_shift) & a.next_mask),
+                               lane);
+                }
+            }
+        }
+
+        // Two or more value arrays: one after the other through the key stage.
+        if constexpr (VMODE == MGPU_VALUES_MULTI) {
+            for (uint32_t v = 0; v < a.num_values; ++v) {
+                load_striped<IPT>(job_vals_in<MULTI>(a, job, v), n, wave_base, lane, val);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
+                __syncthreads();
+                uint32_t* dstv = job_vals_out<MULTI>(a, job, v);
+#pragma unroll
+                for (int i = 0; i < IPT; ++i)
+                    if ((full || i * BLOCK < vrem) && dest[i] < n) dstv[dest[i]] = s_stage[i * BLOCK + tid];
+            }
+        }
+
+        if constexpr (!XSEG) break;
+        __syncthreads();  // every read of this tile's LDS state is done
+        for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) {
+            (&s_count[0][0])[i] = 0;
+            // The value stage wrote over the match masks.
+            if constexpr (RANK == 0 && JOINT) (&s_match[0][0])[i] = 0;
+        }
+    }  // while
+
+    // The next pass's counts of this workgroup's keys, and the last
+    // workgroup to finish checks that every segment was taken to its end
+    // (each needs a workgroup on its XCD).
+    if constexpr (XSEG) {
+        if (count_next) {
+            __syncthreads();
+            for (uint32_t i = tid; i < NSEG * RADIX; i += BLOCK) {
+                const uint32_t c = s_next[i];
+                if (c) __hip_atomic_fetch_add(&a.seg_count_out[i], c, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (tid == 0) {
+            const uint32_t done =
+                __hip_atomic_fetch_add(a.finished, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (done + 1 == gridDim.x) {
+                for (uint32_t x = 0; x < a.num_segs; ++x) {
+                    const uint32_t tiles = mgpu_seg_first_tile(x + 1, a.num_tiles, a.num_segs) -
+                                           mgpu_seg_first_tile(x, a.num_tiles, a.num_segs);
+                    if (ld_agent(a.tile_counter + x) < tiles) st_agent(a.error_flag, 5u);
+                }
+            }
         }
     }
 }
@@ -706,11 +872,15 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
 #define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
-        rank_scatter_body<B, I, R, MODE, false>(a);                                           \
+        rank_scatter_body<B, I, R, MODE, false, false>(a);                                    \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r##R##_x_v##V(mgpuPassArgs a) {                       \
+        rank_scatter_body<B, I, R, MODE, false, true>(a);                                     \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
-        rank_scatter_body<B, I, R, MODE, true>(a);                                            \
+        rank_scatter_body<B, I, R, MODE, true, false>(a);                                     \
     }
 #define MGPU_RANK_SCATTER(B, I, R)                                                            \
     MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
@@ -720,7 +890,8 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
 
 // Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest from
 // 2^22 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps);
-// each with both ranking methods.
+// each with both ranking methods, with one lookback chain across the device
+// (_v) or one per XCD segment (_x_v).
 MGPU_RANK_SCATTER(256, 16, 0)
 MGPU_RANK_SCATTER(512, 16, 0)
 MGPU_RANK_SCATTER(1024, 16, 0)
@@ -744,4 +915,45 @@ extern "C" __global__ void __launch_bounds__(1024) mgpu_rank_probe(uint32_t* out
         out[r * 1024 + tid] = __hip_atomic_fetch_add(&cnt[wave][d], 1u, __ATOMIC_RELAXED,
                                                      __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+}
+
+// Check of what segments per XCD rely on, by one workgroup of 64 lanes per
+// grid slot: lane 0 records the XCD it runs on (out[0 .. grid)), and the
+// first two workgroups to arrive on each XCD ping-pong `rounds` times through
+// a word of that XCD's area with the lookback's nontemporal loads and stores
+// (every wait bounded); out[grid + 2x] = rounds completed on XCD x, out[grid +
+// 2x + 1] = 1 if a wait gave up.  out: grid + 2 * MGPU_MAX_SEGS words; area:
+// zeroed words, MGPU_MAX_SEGS * 128 (the arrival counter and the two flags of
+// an XCD on separate cache lines).
+extern "C" __global__ void __launch_bounds__(64) mgpu_xcd_probe(uint32_t* out, uint32_t* area,
+                                                               uint32_t rounds) {
+    if (threadIdx.x != 0) return;
+    uint32_t xcc;
+    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+    xcc &= 0xFu;
+    out[blockIdx.x] = xcc;
+    uint32_t* base = area + xcc * 128;
+    const uint32_t role =
+        __hip_atomic_fetch_add(base, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (role > 1) return;
+    uint32_t* ping = base + 32;
+    uint32_t* pong = base + 64;
+    uint32_t r = 1;
+    bool failed = false;
+    for (; r <= rounds && !failed; ++r) {
+        uint32_t* wait_on = role == 0 ? pong : ping;
+        if (role == 0) lookback_publish<true>(ping, r);
+        uint32_t spins = 0;
+        while (true) {
+            asm volatile("" ::: "memory");
+            if (lookback_word<true>(wait_on, 0, 0) == r) break;
+            if (++spins > (1u << 20)) {
+                failed = true;
+                break;
+            }
+        }
+        if (role == 1 && !failed) lookback_publish<true>(pong, r);
+    }
+    if (role == 0) out[gridDim.x + 2 * xcc] = failed ? r - 2 : rounds;
+    if (failed) out[gridDim.x + 2 * xcc + 1] = 1;
 }

@@ -354,7 +354,7 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         pa.tile_begin = P.devTileBegin;
         pa.digit_start = digitStart;
         pa.lookback = lane.lookback + (uint64_t)p * P.wideTiles * MGPU_RADIX;
-        pa.tile_counter = lane.tileCounters + p;
+        pa.tile_counter = lane.tileCounters + MGPU_TILE_COUNTER(p, 0);
         pa.error_flag = e->errorFlag;
         pa.num_segs = numWide;
         pa.num_tiles = P.wideTiles;
@@ -504,8 +504,8 @@ sortStatus_t build_orientation(sortSegPlan_d* p, int o) {
     if (st) return st;
     SortLane& lane = e->lane;
     if (lane.histDirty) {  // no memset nodes inside the graph
-        if (hipMemsetAsync(lane.hist, 0, MGPU_MAX_PASSES * MGPU_RADIX * sizeof(uint32_t),
-                           lane.stream) != hipSuccess ||
+        if (hipMemsetAsync(lane.hist, 0, kLaneHistWords * sizeof(uint32_t), lane.stream) !=
+                hipSuccess ||
             hipMemsetAsync(lane.tileCounters, 0, MGPU_LANE_COUNTERS * sizeof(uint32_t),
                            lane.stream) != hipSuccess ||
             hipStreamSynchronize(lane.stream) != hipSuccess)

@@ -91,8 +91,21 @@ def copy_d2d(dst, src, nbytes):
     assert _hip.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, 3) == 0
 
 
-def make_engine():
+def make_engine(env=None):
+    """An engine on torch's current stream; `env` holds MGPUSORT_* settings
+    that the engine reads at creation (restored afterwards)."""
+    import os
     torch = torch_mod()
-    eng = mg.Engine()
+    env = env or {}
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        eng = mg.Engine()
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     eng.set_stream(torch.cuda.current_stream().cuda_stream)
     return eng

@@ -26,6 +26,14 @@ def engine():
     eng.close()
 
 
+@pytest.fixture(scope="module")
+def xcd_engine():
+    """Segments per XCD at every size where each XCD gets a tile."""
+    eng = make_engine({"MGPUSORT_XCD": "1"})
+    yield eng
+    eng.close()
+
+
 def _check_vs_oracle(oracle, keys, g, values=(), value_count=None, fb=0, eb=32):
     vc = len(values) if value_count is None else value_count
     rk, rv = oracle.stable_sort(keys, list(values), value_count=vc, first_bit=fb, end_bit=eb)
@@ -239,12 +247,16 @@ def test_sort_host_and_device(engine, oracle):
 
 
 @pytest.mark.parametrize("kind,executed", [("random", 4), ("sorted", 0), ("top_constant", 3),
-                                           ("low_constant", 3), ("two_constant", 2)])
+                                           ("low_constant", 3), ("two_constant", 2),
+                                           ("mid_constant", 3)])
 @pytest.mark.parametrize("vc", [-1, 1])
-def test_early_exit(engine, oracle, kind, executed, vc):
+@pytest.mark.parametrize("which", ["default", "xcd"])
+def test_early_exit(request, oracle, kind, executed, vc, which):
     """earlyExit skips passes whose digit is the same for every key, and every
     pass of already-sorted input; the output is unchanged and parity follows
-    the executed passes."""
+    the executed passes (with segments per XCD, each executed pass counts
+    the segments of the next executed one)."""
+    engine = request.getfixturevalue("engine" if which == "default" else "xcd_engine")
     n = 150_001
     keys = workloads.uniform_keys(n, seed=90)
     if kind == "sorted":
@@ -255,6 +267,8 @@ def test_early_exit(engine, oracle, kind, executed, vc):
         keys = (keys & np.uint32(0xFFFFFF00)) | np.uint32(0x33)
     elif kind == "two_constant":
         keys = (keys & np.uint32(0x00FFFF00)) | np.uint32(0x77000011)
+    elif kind == "mid_constant":  # pass 0 is followed by pass 2
+        keys = (keys & np.uint32(0xFFFF00FF)) | np.uint32(0x00004400)
     vals = [np.arange(n, dtype=np.uint32)] if vc == 1 else []
     g = GpuSort(keys, vals, value_count=vc)
     g.data.earlyExit = True
@@ -263,6 +277,19 @@ def test_early_exit(engine, oracle, kind, executed, vc):
     _check_vs_oracle(oracle, keys, g, vals, vc)
     assert g.data.parity == executed % 2
     assert g.data.keys[g.data.parity] == g.start_keys
+
+
+def test_engine_info(engine):
+    """The engine's creation-time choices on a gfx950 device: the atomic-add
+    ranking and one lookback chain per XCD (MI355X: 8 XCDs), with no device
+    inconsistency recorded."""
+    info = engine.info()
+    assert info["numCUs"] > 0
+    assert info["rankMethod"] in (0, 1)
+    assert info["xcdSegments"] == 1 or 2 <= info["xcdSegments"] <= 8
+    assert info["smallTileKeys"] == 512 * 16 and info["largeTileKeys"] == 1024 * 16
+    assert info["deviceError"] == 0
+    print("engine info:", info, flush=True)
 
 
 def test_keys_past_the_end_are_untouched(engine):

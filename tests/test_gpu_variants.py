@@ -1,9 +1,12 @@
-"""Every built rank_scatter tile shape, with either in-wave ranking method,
-gives the same bit-exact result as the oracle, for plain and multi-array
-(segmented large) sorts.  Read at engine creation: MGPUSORT_TILE forces one
-shape for every size; MGPUSORT_RANK one ranking method (0: LDS match masks;
-1: one LDS atomic add per key, which the engine otherwise takes only after
-checking that the device orders same-word lanes)."""
+"""Every built rank_scatter tile shape, with either in-wave ranking method and
+either lookback layout, gives the same bit-exact result as the oracle, for
+plain and multi-array (segmented large) sorts.  Read at engine creation:
+MGPUSORT_TILE forces one shape for every size; MGPUSORT_RANK one ranking
+method (0: LDS match masks; 1: one LDS atomic add per key, which the engine
+otherwise takes only after checking that the device orders same-word lanes);
+MGPUSORT_XCD=0 keeps one lookback chain across the device, MGPUSORT_XCD=1
+takes one per XCD segment at every size (the engine otherwise takes them,
+after its own check, for large sorts only)."""
 import os
 
 import numpy as np
@@ -15,15 +18,16 @@ from gpu_helpers import GpuSort, make_engine
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [(t, r) for r in ("0", "1") for t in ("256x16", "512x16", "1024x16")]
-ENV = ("MGPUSORT_TILE", "MGPUSORT_RANK")
+VARIANTS = [(t, r, x) for x in ("1", "0") for r in ("0", "1")
+            for t in ("256x16", "512x16", "1024x16")]
+ENV = ("MGPUSORT_TILE", "MGPUSORT_RANK", "MGPUSORT_XCD")
 
 
-@pytest.fixture(params=VARIANTS, ids=[f"{t}_r{r}" for t, r in VARIANTS])
+@pytest.fixture(params=VARIANTS, ids=[f"{t}_r{r}_xcd{x}" for t, r, x in VARIANTS])
 def variant_engine(request):
-    tile, rank = request.param
+    tile, rank, xcd = request.param
     saved = {k: os.environ.get(k) for k in ENV}
-    os.environ.update(MGPUSORT_TILE=tile, MGPUSORT_RANK=rank)
+    os.environ.update(MGPUSORT_TILE=tile, MGPUSORT_RANK=rank, MGPUSORT_XCD=xcd)
     try:
         eng = make_engine()
     finally:
@@ -36,7 +40,7 @@ def variant_engine(request):
     eng.close()
 
 
-@pytest.mark.parametrize("n", [1, 5000, 16384, 16385, 1_000_003])
+@pytest.mark.parametrize("n", [1, 5000, 16384, 16385, 1_000_003, 4_194_305])
 @pytest.mark.parametrize("vc", [-1, 1, 3])
 def test_variant_matches_oracle(variant_engine, oracle, n, vc):
     keys = workloads.and3_keys(n, seed=n + vc)
