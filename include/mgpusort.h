@@ -284,8 +284,9 @@ sortStatus_t SORTAPI sortServerReplay(sortServer_t server, const sortRequest_d* 
 sortStatus_t SORTAPI sortServerDestroy(sortServer_t server);
 
 /* Waits for the engine's stream and reports SORT_STATUS_KERNEL_ERROR if a
- * kernel detected an inconsistency (a scatter offset out of range, or a
- * lookback wait that gave up) since the last call; SUCCESS otherwise. */
+ * kernel detected an inconsistency (a scatter offset out of range, a
+ * lookback wait that gave up, or a segment left without a workgroup on its
+ * XCD) since the last call; SUCCESS otherwise. */
 sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine);
 
 /* What the engine chose at creation, and the last device inconsistency code

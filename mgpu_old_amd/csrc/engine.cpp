@@ -365,6 +365,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
             if (nextRun < numPasses) {
                 pa.seg_count_out =
                     L.segCount + (size_t)nextRun * MGPU_XSEG_COUNT_SEGS * MGPU_RADIX;
+                pa.next_mode = L.tileCounters + MGPU_MODE_SLOT + nextRun;
                 pa.next_shift = ca.shift[nextRun];
                 pa.next_mask = ca.mask[nextRun];
             }

@@ -105,7 +105,9 @@ struct mgpuScanArgs {
 #define MGPU_TILE_COUNTER(p, x) ((p) * MGPU_MAX_SEGS + (x))
 #define MGPU_FINISHED_SLOT (MGPU_MAX_PASSES * MGPU_MAX_SEGS)
 #define MGPU_UNSORTED_SLOT (MGPU_FINISHED_SLOT + MGPU_MAX_PASSES)
-#define MGPU_LANE_COUNTERS (MGPU_UNSORTED_SLOT + 4)
+// Most frequent digit of each pass (segments per XCD only; digit_scan).
+#define MGPU_MODE_SLOT (MGPU_UNSORTED_SLOT + 4)
+#define MGPU_LANE_COUNTERS (MGPU_MODE_SLOT + MGPU_MAX_PASSES)
 
 // iota: out[i] = i for i < n (index output of a sort that needs no pass).
 struct mgpuIotaArgs {
@@ -220,6 +222,7 @@ struct mgpuPassArgs {
     // seg_count_out ([MGPU_XSEG_COUNT_SEGS][MGPU_RADIX], zero on entry).
     const uint32_t* seg_count_in;
     uint32_t* seg_count_out;
+    const uint32_t* next_mode;  // most frequent digit of the next pass (with seg_count_out)
     uint32_t next_shift;
     uint32_t next_mask;
 };

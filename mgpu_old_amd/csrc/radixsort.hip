@@ -60,6 +60,18 @@ __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x, uint32_t lan
     return x;
 }
 
+// Maximum over the 64 lanes of a wave.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)v, off, 64);
+        const uint32_t hi = __shfl_xor((uint32_t)(v >> 32), off, 64);
+        const uint64_t o = ((uint64_t)hi << 32) | lo;
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
 // Exclusive scan of x over the first 256 threads of the block (threads past
 // 256 must pass 0).  s_tmp holds one word per wave.  Contains barriers: every
 // thread of the block must call it.
@@ -104,16 +116,21 @@ __device__ __forceinline__ void count4(uint32_t* h, uint32_t k0, uint32_t k1, ui
 }
 
 // Adds 1 per active lane to LDS counter h[i], without branches.  The lanes of
-// one LDS atomic that hit the same word are serialized, so when every active
-// lane has the same i the first active lane adds their number and the others
-// add 0 to a word of their own in spare[64].
-__device__ __forceinline__ void count1(uint32_t* h, uint32_t* spare, uint32_t i, uint32_t lane) {
-    const uint32_t f = __builtin_amdgcn_readfirstlane(i);
+// one LDS atomic that hit the same word are serialized, so one group of lanes
+// with equal i is merged first: the lanes with the first `pref` lane's i (the
+// caller's guess at the most frequent value), else with the first active
+// lane's i.  The group's first lane adds the group's size; the others add 0
+// to a word of their own in spare[64]; every other lane adds 1.
+__device__ __forceinline__ void count1(uint32_t* h, uint32_t* spare, uint32_t i, bool pref,
+                                       uint32_t lane) {
     const uint64_t exec = __builtin_amdgcn_read_exec();
-    const bool one = __builtin_amdgcn_ballot_w64(i == f) == exec;
-    const bool first = lanes_below(exec) == 0;
-    uint32_t* const w = one && !first ? &spare[lane] : &h[i];
-    const uint32_t v = one ? (first ? (uint32_t)__builtin_popcountll(exec) : 0u) : 1u;
+    const uint64_t pm = __builtin_amdgcn_ballot_w64(pref);
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pm ? pm : exec);
+    const uint32_t f = __builtin_amdgcn_readlane(i, leader);
+    const uint64_t m = __builtin_amdgcn_ballot_w64(i == f);
+    const bool peer = i == f && lane != leader;
+    uint32_t* const w = peer ? &spare[lane] : &h[i];
+    const uint32_t v = peer ? 0u : (lane == leader ? (uint32_t)__builtin_popcountll(m) : 1u);
     __hip_atomic_fetch_add(w, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -228,11 +245,22 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
 // digit d in segments before x.
 __device__ void digit_scan_body(const mgpuScanArgs& a) {
     __shared__ uint32_t s_tmp[4];
+    __shared__ uint64_t s_max[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (uint32_t p = 0; p < a.num_passes; ++p) {
         uint32_t total = 0;
         for (uint32_t x = 0; x < a.num_segs; ++x)
             total += a.hist[(x * MGPU_MAX_PASSES + p) * RADIX + tid];
+        if (a.seg_count_words) {  // segments per XCD: the pass's most frequent digit
+            const uint64_t best = wave_max_u64(((uint64_t)total << 8) | tid);
+            if (lane == 0) s_max[wave] = best;
+            __syncthreads();
+            if (tid == 0) {
+                uint64_t m = s_max[0];
+                for (int w = 1; w < 4; ++w) m = s_max[w] > m ? s_max[w] : m;
+                a.tile_counters[MGPU_MODE_SLOT + p] = (uint32_t)(m & 0xFFu);
+            }
+        }
         uint32_t run = block_exclusive_scan<256>(total, s_tmp, lane, wave);
         for (uint32_t x = 0; x < a.num_segs; ++x) {
             const uint32_t i = (x * MGPU_MAX_PASSES + p) * RADIX + tid;
@@ -242,7 +270,7 @@ __device__ void digit_scan_body(const mgpuScanArgs& a) {
             a.hist[i] = 0;
         }
     }
-    for (uint32_t i = tid; i < MGPU_LANE_COUNTERS; i += 256) a.tile_counters[i] = 0;
+    for (uint32_t i = tid; i < MGPU_MODE_SLOT; i += 256) a.tile_counters[i] = 0;
     for (uint32_t i = tid; i < a.seg_count_words; i += 256) a.seg_count[i] = 0;
 }
 
@@ -576,14 +604,17 @@ __device__ void rank_scatter_body(const Args& a) {
     // never), whether this pass counts for it, and this segment's starts.
     uint32_t next_bound[NSEG];
     bool count_next = false;
+    uint32_t next_mode = RADIX;  // most frequent digit of the next pass
     if constexpr (XSEG) {
 #pragma unroll
         for (uint32_t x = 0; x < NSEG; ++x)
             next_bound[x] = x < a.num_segs ? mgpu_seg_first_tile(x, a.num_tiles, a.num_segs) * TILE
                                            : 0xFFFFFFFFu;
         count_next = a.seg_count_out != nullptr;
-        if (count_next)
+        if (count_next) {
+            next_mode = *a.next_mode;
             for (uint32_t i = tid; i < NSEG * RADIX; i += BLOCK) s_next[i] = 0;
+        }
         if (tid < RADIX && seg < a.num_segs) {
             uint32_t st;
             if (a.seg_count_in) {
@@ -780,16 +811,19 @@ __device__ void rank_scatter_body(const Args& a) {
         if (bad) st_agent(a.error_flag, 4u);
         // Segments per XCD: the next pass's digit of every key written, by
         // the segment of the next pass that its position falls in.
+        // (Finding the segment by comparing with the uniform bounds costs
+        // less than a per-digit LDS lookup.)
         if constexpr (XSEG) {
             if (count_next) {
 #pragma unroll
                 for (int i = 0; i < IPT; ++i) {
-                    uint32_t xo = 0;
+                    if (dest[i] < n) {
+                        uint32_t xo = 0;
 #pragma unroll
-                    for (uint32_t x = 1; x < NSEG; ++x) xo += dest[i] >= next_bound[x];
-                    if (dest[i] < n)
-                        count1(s_next, s_spare, xo * RADIX + ((kout[i] >> a.next_shift) & a.next_mask),
-                               lane);
+                        for (uint32_t x = 1; x < NSEG; ++x) xo += dest[i] >= next_bound[x];
+                        const uint32_t nd = (kout[i] >> a.next_shift) & a.next_mask;
+                        count1(s_next, s_spare, xo * RADIX + nd, nd == next_mode, lane);
+                    }
                 }
             }
         }
