@@ -251,11 +251,15 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         ca.check_shift = (uint32_t)firstBit;
         ca.check_mask = totalBits >= 32 ? 0xffffffffu : ((1u << totalBits) - 1u);
     }
-    // Chunks of about 8192 keys, at most 8 workgroups per CU, cut per segment.
+    // Chunks cut per segment: of about 8192 keys for the 256-thread kernel (at
+    // most 8 workgroups per CU), 32768 for the lane-column one (one per CU).
     ca.num_segs = segs;
     ca.tile = tile;
     ca.num_tiles = numTiles;
-    const uint32_t maxGrid = (uint32_t)e->numCUs * 8;
+    const bool cols = n >= kCountColsMinKeys;
+    const uint32_t countThreads = cols ? 1024u : 256u;
+    const uint32_t chunkTarget = cols ? 32768u : 8192u;
+    const uint32_t maxGrid = (uint32_t)e->numCUs * (cols ? 1u : 8u);
     uint64_t segKeys = 0;  // keys of the largest segment
     for (uint32_t x = 0; x < segs; ++x) {
         const uint64_t b = (uint64_t)mgpu_seg_first_tile(x, numTiles, segs) * tile;
@@ -263,7 +267,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
             x + 1 == segs ? n : (uint64_t)mgpu_seg_first_tile(x + 1, numTiles, segs) * tile;
         segKeys = std::max(segKeys, en - b);
     }
-    uint32_t perSeg = (uint32_t)std::max<uint64_t>(1, (segKeys + 8191) / 8192);
+    uint32_t perSeg = (uint32_t)std::max<uint64_t>(1, (segKeys + chunkTarget - 1) / chunkTarget);
     perSeg = std::max(1u, std::min(perSeg, maxGrid / segs));
     ca.chunk = (uint32_t)((segKeys + perSeg - 1) / perSeg);
     ca.chunk = (ca.chunk + 1023) & ~1023u;
@@ -271,14 +275,15 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     uint32_t grid = ca.chunks_per_seg * segs;
     // Enough workgroups to zero the lookback words quickly as well (the
     // extra ones find their chunk empty).
-    uint32_t clearGrid = (uint32_t)((ca.clear_words / 4 + 256 * 16 - 1) / (256 * 16));
+    uint32_t clearGrid =
+        (uint32_t)((ca.clear_words / 4 + countThreads * 16 - 1) / (countThreads * 16));
     if (clearGrid > maxGrid) clearGrid = maxGrid;
     if (clearGrid > grid) {
         ca.chunks_per_seg = (clearGrid + segs - 1) / segs;
         grid = ca.chunks_per_seg * segs;
     }
     L.histDirty = true;
-    st = mgpu_launch(e, L.stream, e->fnCount, grid, 256, &ca);
+    st = mgpu_launch(e, L.stream, cols ? e->fnCountCols : e->fnCount, grid, countThreads, &ca);
     if (st) return st;
 
     // Early exit (the role of the reference's sortDetectCounters readback,

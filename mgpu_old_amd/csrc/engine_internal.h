@@ -62,7 +62,8 @@ struct sortEngine_d {
     SortLane lane;  // default lane: sortArray and friends run here
 
     hipModule_t radixModule = nullptr;
-    hipFunction_t fnCount = nullptr;
+    hipFunction_t fnCount = nullptr;      // 256 threads, per-wave histograms
+    hipFunction_t fnCountCols = nullptr;  // 1024 threads, lane-column histograms (large sorts)
     hipFunction_t fnScan = nullptr;
     hipFunction_t fnIota = nullptr;
     hipFunction_t fnMultiCount = nullptr;
@@ -119,6 +120,9 @@ constexpr uint32_t kLargeTilesPerCU = 1;
 // next pass are flushed once); they win from about this many tiles per XCD
 // (2^24 keys in 16384-key tiles on 8 XCDs).
 constexpr uint32_t kXsegMinTilesPerSeg = 128;
+// digit_count with lane-column histograms (one 1024-thread workgroup per CU)
+// from this many keys up.
+constexpr uint32_t kCountColsMinKeys = 1u << 22;
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys);
 
 // Grows the lane's lookback words to cover a device-wide sort of n keys in

@@ -132,6 +132,7 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
         return SORT_STATUS_KERNEL_NOT_FOUND;
     }
     sortStatus_t st = get_fn(e, e->radixModule, "mgpu_digit_count", &e->fnCount);
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_count_cols", &e->fnCountCols);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);

@@ -135,16 +135,28 @@ __device__ __forceinline__ void count1(uint32_t* h, uint32_t* spare, uint32_t i,
 }
 
 // ---------------------------------------------------------------------------
-// digit_count: per-wave LDS histograms of all passes, merged into the global
+// digit_count: LDS histograms of all passes' digits, merged into the global
 // histogram with one atomic per (pass, digit) per workgroup.  Also zeroes the
 // lookback words the passes of this sort will use.
-template <int BLOCK>
+//
+// COLS = 0: one histogram per wave ([wave][pass][digit]); a wave adds 4 keys
+// per lane with one atomic when all of them share a digit (count4).
+// COLS = 32: one histogram per workgroup with a column per lane slot
+// ([pass][digit][lane % 32]): the 32 lanes of a half-wave always hit different
+// LDS banks, and lanes with equal digits different words, so an LDS atomic
+// never serializes more than the two lanes of a wave that share a column (an
+// atomic add per key costs ~11 CU cycles per wave on random banks, ~5.6 on
+// distinct ones, ~54 when a third of the lanes share one word).  128 KB of
+// LDS: 1024 threads, one workgroup per CU.
+template <int BLOCK, int COLS>
 __device__ void digit_count_body(const mgpuCountArgs& a) {
     constexpr int W = BLOCK / 64;
-    __shared__ uint32_t s_hist[W][MGPU_MAX_PASSES][RADIX];
+    constexpr uint32_t WORDS = COLS ? MGPU_MAX_PASSES * RADIX * COLS : W * MGPU_MAX_PASSES * RADIX;
+    static_assert(COLS == 0 || (COLS == 32 && BLOCK == MGPU_MAX_PASSES * RADIX),
+                  "the column layout merges one (pass, digit) per thread");
+    __shared__ uint32_t s_hist[WORDS];
     const uint32_t tid = threadIdx.x, wave = tid >> 6;
-    uint32_t* flat = &s_hist[0][0][0];
-    for (uint32_t i = tid; i < (uint32_t)(W * MGPU_MAX_PASSES * RADIX); i += BLOCK) flat[i] = 0;
+    for (uint32_t i = tid; i < WORDS; i += BLOCK) s_hist[i] = 0;
 
     // Zero this sort's lookback words (grid-stride, 16-byte stores).
     {
@@ -156,14 +168,21 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
     }
     __syncthreads();
 
+    // Word of (pass, digit) for this thread: pass p's histogram starts at
+    // p * PSTRIDE, digits are DSTRIDE apart, and the thread's own histogram
+    // (COLS = 0) or column (COLS = 32) is at `mine`.
+    constexpr uint32_t PSTRIDE = COLS ? RADIX * COLS : RADIX;
+    constexpr uint32_t DSTRIDE = COLS ? COLS : 1;
+    uint32_t* const mine = COLS ? s_hist + (tid & (COLS - 1)) : s_hist + wave * MGPU_MAX_PASSES * RADIX;
     const uint32_t np = a.num_passes;
-    uint32_t (*h)[RADIX] = s_hist[wave];
+    auto add = [&](uint32_t p, uint32_t d) {
+        __hip_atomic_fetch_add(&mine[p * PSTRIDE + d * DSTRIDE], 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
     auto count_key = [&](uint32_t k) {
 #pragma unroll
         for (int p = 0; p < MGPU_MAX_PASSES; ++p)
-            if ((uint32_t)p < np)
-                __hip_atomic_fetch_add(&h[p][(k >> a.shift[p]) & a.mask[p]], 1u, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+            if ((uint32_t)p < np) add(p, (k >> a.shift[p]) & a.mask[p]);
     };
 
     // Chunk of this workgroup inside its segment; the segment's histograms.
@@ -201,9 +220,18 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                     const uint32_t i = i0 + u * BLOCK;
                     if (i >= nvec) break;
 #pragma unroll
-                    for (int p = 0; p < MGPU_MAX_PASSES; ++p)
-                        if ((uint32_t)p < np)
-                            count4(h[p], q[u].x, q[u].y, q[u].z, q[u].w, a.shift[p], a.mask[p]);
+                    for (int p = 0; p < MGPU_MAX_PASSES; ++p) {
+                        if ((uint32_t)p >= np) continue;
+                        if constexpr (COLS == 0) {
+                            count4(mine + p * PSTRIDE, q[u].x, q[u].y, q[u].z, q[u].w, a.shift[p],
+                                   a.mask[p]);
+                        } else {
+                            add(p, (q[u].x >> a.shift[p]) & a.mask[p]);
+                            add(p, (q[u].y >> a.shift[p]) & a.mask[p]);
+                            add(p, (q[u].z >> a.shift[p]) & a.mask[p]);
+                            add(p, (q[u].w >> a.shift[p]) & a.mask[p]);
+                        }
+                    }
                     if (check) {
                         descents += (bf(q[u].x) > bf(q[u].y)) + (bf(q[u].y) > bf(q[u].z)) +
                                     (bf(q[u].z) > bf(q[u].w));
@@ -229,12 +257,25 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
         __hip_atomic_fetch_add(a.unsorted, descents, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
 
-    for (uint32_t i = tid; i < (uint32_t)(MGPU_MAX_PASSES * RADIX); i += BLOCK) {
-        if (i / RADIX >= np) break;
-        uint32_t s = 0;
+    if constexpr (COLS == 0) {
+        for (uint32_t i = tid; i < (uint32_t)(MGPU_MAX_PASSES * RADIX); i += BLOCK) {
+            if (i / RADIX >= np) break;
+            uint32_t sum = 0;
 #pragma unroll
-        for (int w = 0; w < W; ++w) s += s_hist[w][i / RADIX][i % RADIX];
-        if (s) __hip_atomic_fetch_add(&hist[i], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            for (int w = 0; w < W; ++w) sum += s_hist[w * MGPU_MAX_PASSES * RADIX + i];
+            if (sum) __hip_atomic_fetch_add(&hist[i], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else {
+        // Thread i sums the columns of (pass, digit) i, starting at column i
+        // so that the lanes of a wave read different banks.
+        const uint32_t i = tid;
+        if (i / RADIX < np) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t c = 0; c < (uint32_t)COLS; ++c)
+                sum += s_hist[i * COLS + ((c + i) & (COLS - 1))];
+            if (sum) __hip_atomic_fetch_add(&hist[i], sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -884,7 +925,11 @@ __device__ void rank_scatter_body(const Args& a) {
 // Exported entry points (names looked up by the engine).
 
 extern "C" __global__ void __launch_bounds__(256) mgpu_digit_count(mgpuCountArgs a) {
-    digit_count_body<256>(a);
+    digit_count_body<256, 0>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(1024) mgpu_digit_count_cols(mgpuCountArgs a) {
+    digit_count_body<1024, 32>(a);
 }
 
 extern "C" __global__ void __launch_bounds__(256) mgpu_digit_scan(mgpuScanArgs a) {
