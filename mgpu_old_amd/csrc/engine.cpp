@@ -326,7 +326,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     sa.seg_count = L.segCount;
     sa.seg_count_words = segs > 1 ? (uint32_t)kLaneSegCountWords : 0u;
     sa.pad_ = 0;
-    st = mgpu_launch(e, L.stream, e->fnScan, 1, 256, &sa);
+    st = mgpu_launch(e, L.stream, e->fnScan, (unsigned)numPasses, 256, &sa);
     if (st) return st;
     L.histDirty = false;
 

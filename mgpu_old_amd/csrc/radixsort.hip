@@ -280,39 +280,45 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
 }
 
 // ---------------------------------------------------------------------------
-// digit_scan: one workgroup of 256 threads (thread d: digit d).  A pass's
-// output holds the digits in order and, within a digit, the segments in
-// order: run (x, d) starts at the keys of smaller digits plus the keys of
-// digit d in segments before x.
+// digit_scan: one workgroup of 256 threads per pass (workgroup p: pass p;
+// thread d: digit d).  A pass's output holds the digits in order and, within
+// a digit, the segments in order: run (x, d) starts at the keys of smaller
+// digits plus the keys of digit d in segments before x.
 __device__ void digit_scan_body(const mgpuScanArgs& a) {
     __shared__ uint32_t s_tmp[4];
     __shared__ uint64_t s_max[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (uint32_t p = 0; p < a.num_passes; ++p) {
-        uint32_t total = 0;
-        for (uint32_t x = 0; x < a.num_segs; ++x)
-            total += a.hist[(x * MGPU_MAX_PASSES + p) * RADIX + tid];
-        if (a.seg_count_words) {  // segments per XCD: the pass's most frequent digit
-            const uint64_t best = wave_max_u64(((uint64_t)total << 8) | tid);
-            if (lane == 0) s_max[wave] = best;
-            __syncthreads();
-            if (tid == 0) {
-                uint64_t m = s_max[0];
-                for (int w = 1; w < 4; ++w) m = s_max[w] > m ? s_max[w] : m;
-                a.tile_counters[MGPU_MODE_SLOT + p] = (uint32_t)(m & 0xFFu);
-            }
+    const uint32_t p = blockIdx.x;
+    uint32_t c[MGPU_MAX_SEGS];
+    uint32_t total = 0;
+#pragma unroll
+    for (uint32_t x = 0; x < MGPU_MAX_SEGS; ++x) {
+        c[x] = x < a.num_segs ? a.hist[(x * MGPU_MAX_PASSES + p) * RADIX + tid] : 0u;
+        total += c[x];
+    }
+    if (a.seg_count_words) {  // segments per XCD: the pass's most frequent digit
+        const uint64_t best = wave_max_u64(((uint64_t)total << 8) | tid);
+        if (lane == 0) s_max[wave] = best;
+        __syncthreads();
+        if (tid == 0) {
+            uint64_t m = s_max[0];
+            for (int w = 1; w < 4; ++w) m = s_max[w] > m ? s_max[w] : m;
+            a.tile_counters[MGPU_MODE_SLOT + p] = (uint32_t)(m & 0xFFu);
         }
-        uint32_t run = block_exclusive_scan<256>(total, s_tmp, lane, wave);
-        for (uint32_t x = 0; x < a.num_segs; ++x) {
+    }
+    uint32_t run = block_exclusive_scan<256>(total, s_tmp, lane, wave);
+#pragma unroll
+    for (uint32_t x = 0; x < MGPU_MAX_SEGS; ++x) {
+        if (x < a.num_segs) {
             const uint32_t i = (x * MGPU_MAX_PASSES + p) * RADIX + tid;
-            const uint32_t c = a.hist[i];
             a.digit_start[i] = run;
-            run += c;
+            run += c[x];
             a.hist[i] = 0;
         }
     }
-    for (uint32_t i = tid; i < MGPU_MODE_SLOT; i += 256) a.tile_counters[i] = 0;
-    for (uint32_t i = tid; i < a.seg_count_words; i += 256) a.seg_count[i] = 0;
+    if (p == 0)
+        for (uint32_t i = tid; i < MGPU_MODE_SLOT; i += 256) a.tile_counters[i] = 0;
+    for (uint32_t i = p * 256 + tid; i < a.seg_count_words; i += gridDim.x * 256) a.seg_count[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
