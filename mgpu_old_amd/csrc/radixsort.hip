@@ -115,17 +115,20 @@ __device__ __forceinline__ void count4(uint32_t* h, uint32_t k0, uint32_t k1, ui
     __hip_atomic_fetch_add(&h[d3], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Adds 1 per active lane to LDS counter h[i], without branches.  The lanes of
-// one LDS atomic that hit the same word are serialized, so one group of lanes
-// with equal i is merged first: the lanes with the first `pref` lane's i (the
-// caller's guess at the most frequent value), else with the first active
-// lane's i.  The group's first lane adds the group's size; the others add 0
-// to a word of their own in spare[64]; every other lane adds 1.
+// Adds 1 per active lane to LDS counter h[i].  The lanes of one LDS atomic
+// that hit the same word are serialized, so when some lanes hold `pref` (the
+// caller's guess at the most frequent value) the lanes with the first such
+// lane's i are merged first: that lane adds the group's size, the others add
+// 0 to a word of their own in spare[64], every other lane adds 1.  Otherwise
+// (the common case on spread-out input) each lane adds 1.
 __device__ __forceinline__ void count1(uint32_t* h, uint32_t* spare, uint32_t i, bool pref,
                                        uint32_t lane) {
-    const uint64_t exec = __builtin_amdgcn_read_exec();
     const uint64_t pm = __builtin_amdgcn_ballot_w64(pref);
-    const uint32_t leader = (uint32_t)__builtin_ctzll(pm ? pm : exec);
+    if (pm == 0) {
+        __hip_atomic_fetch_add(&h[i], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return;
+    }
+    const uint32_t leader = (uint32_t)__builtin_ctzll(pm);
     const uint32_t f = __builtin_amdgcn_readlane(i, leader);
     const uint64_t m = __builtin_amdgcn_ballot_w64(i == f);
     const bool peer = i == f && lane != leader;
@@ -574,11 +577,17 @@ __device__ __forceinline__ uint32_t* job_vals_out(const Args& a, const TileJob& 
 constexpr int kBufferDword3 = 0x00020000;  // 32-bit data format, no swizzle
 constexpr int kLoadNontemporal = 2;
 
+// Buffer resource over an array of n words (n < 2^30, so byte offsets fit in
+// 32 bits): loads at or past n return 0, stores there are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t array_rsrc(const uint32_t* p, uint32_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(p), 0, (int)(n * 4u),
+                                             kBufferDword3);
+}
+
 template <int IPT>
 __device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t n, uint32_t wave_base,
                                              uint32_t lane, uint32_t* out) {
-    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<uint32_t*>(src), 0, (int)(n * 4u), kBufferDword3);
+    const __amdgpu_buffer_rsrc_t r = array_rsrc(src, n);
     const uint32_t off = (wave_base + lane) * 4u;
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
@@ -647,16 +656,18 @@ __device__ void rank_scatter_body(const Args& a) {
         (&s_count[0][0])[i] = 0;
         if constexpr (RANK == 0) (&s_match[0][0])[i] = 0;
     }
-    // XSEG: where the next pass's segments start (uniform; past num_segs:
-    // never), whether this pass counts for it, and this segment's starts.
-    uint32_t next_bound[NSEG];
+    // XSEG: whether this pass counts for the next one, and this segment's
+    // digit starts.
     bool count_next = false;
+    // Segment of output position q: ((q / TILE + 1) * S - 1) / T for S
+    // segments of T tiles (the largest x with floor(x * T / S) * TILE <= q),
+    // as fma(num, 1/T, 0.5/T) truncated: num < 2^21 is exact in a float, the
+    // quotient is at least 0.5/T >= 2^-17 from an integer, and the rounding of
+    // 1/T and of the fma errs by less than 2^-19.
+    const float seg_rcp = 1.0f / (float)a.num_tiles;
+    const float seg_half = 0.5f * seg_rcp;
     uint32_t next_mode = RADIX;  // most frequent digit of the next pass
     if constexpr (XSEG) {
-#pragma unroll
-        for (uint32_t x = 0; x < NSEG; ++x)
-            next_bound[x] = x < a.num_segs ? mgpu_seg_first_tile(x, a.num_tiles, a.num_segs) * TILE
-                                           : 0xFFFFFFFFu;
         count_next = a.seg_count_out != nullptr;
         if (count_next) {
             next_mode = *a.next_mode;
@@ -672,8 +683,6 @@ __device__ void rank_scatter_body(const Args& a) {
             }
             s_dstart[tid] = st;
         }
-    } else {
-        (void)next_bound;
     }
 
     while (true) {
@@ -842,32 +851,35 @@ __device__ void rank_scatter_body(const Args& a) {
             if constexpr (JOINT) vout[i] = s_vstage[i * BLOCK + tid];
         }
         // Offsets outside [0, n) can only come from a broken count or
-        // lookback: they are flagged instead of written (dest = n).
+        // lookback: they are flagged instead of written.  Slots with nothing
+        // to write get dest = n, where the buffer stores' range check drops
+        // them (one 32-bit offset per key instead of 64-bit addresses and a
+        // branch).
+        const __amdgpu_buffer_rsrc_t rk = array_rsrc(job.keys_out, n);
+        const __amdgpu_buffer_rsrc_t rv = JOINT ? array_rsrc(vout0, n) : rk;
         bool bad = false;
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
             const bool real = full || i * BLOCK < vrem;
             const bool ok = real && dest[i] < n;
             bad = bad || (real && !ok);
-            if (!ok) dest[i] = n;
-            if (ok) {
-                job.keys_out[dest[i]] = kout[i];
-                if constexpr (JOINT) vout0[dest[i]] = vout[i];
-            }
+            dest[i] = ok ? dest[i] : n;
+            __builtin_amdgcn_raw_buffer_store_b32(kout[i], rk, (int)(dest[i] * 4u), 0, 0);
+            if constexpr (JOINT)
+                __builtin_amdgcn_raw_buffer_store_b32(vout[i], rv, (int)(dest[i] * 4u), 0, 0);
         }
         if (bad) st_agent(a.error_flag, 4u);
         // Segments per XCD: the next pass's digit of every key written, by
         // the segment of the next pass that its position falls in.
-        // (Finding the segment by comparing with the uniform bounds costs
-        // less than a per-digit LDS lookup.)
+        // (Computing the segment costs less than a per-digit LDS lookup.)
         if constexpr (XSEG) {
             if (count_next) {
 #pragma unroll
                 for (int i = 0; i < IPT; ++i) {
                     if (dest[i] < n) {
-                        uint32_t xo = 0;
-#pragma unroll
-                        for (uint32_t x = 1; x < NSEG; ++x) xo += dest[i] >= next_bound[x];
+                        const uint32_t num = (dest[i] / TILE) * a.num_segs + a.num_segs - 1;
+                        const uint32_t xo =
+                            (uint32_t)__builtin_fmaf((float)num, seg_rcp, seg_half);
                         const uint32_t nd = (kout[i] >> a.next_shift) & a.next_mask;
                         count1(s_next, s_spare, xo * RADIX + nd, nd == next_mode, lane);
                     }
