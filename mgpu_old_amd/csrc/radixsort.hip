@@ -638,6 +638,7 @@ __device__ void rank_scatter_body(const Args& a) {
     const uint32_t lane = tid & 63;
     const uint32_t wave = tid >> 6;
     const uint32_t shift = a.shift, mask = a.mask;
+    const uint32_t width = (uint32_t)__builtin_popcount(mask);  // mask = 2^width - 1
 
     // The segment of this workgroup's XCD (XSEG), else the whole array.
     uint32_t seg = 0, seg_first = 0, seg_tiles = a.num_tiles;
@@ -667,10 +668,12 @@ __device__ void rank_scatter_body(const Args& a) {
     const float seg_rcp = 1.0f / (float)a.num_tiles;
     const float seg_half = 0.5f * seg_rcp;
     uint32_t next_mode = RADIX;  // most frequent digit of the next pass
+    uint32_t next_width = 0;
     if constexpr (XSEG) {
         count_next = a.seg_count_out != nullptr;
         if (count_next) {
             next_mode = *a.next_mode;
+            next_width = (uint32_t)__builtin_popcount(a.next_mask);
             for (uint32_t i = tid; i < NSEG * RADIX; i += BLOCK) s_next[i] = 0;
         }
         if (tid < RADIX && seg < a.num_segs) {
@@ -724,13 +727,15 @@ __device__ void rank_scatter_body(const Args& a) {
         if constexpr (VMODE == MGPU_VALUES_SINGLE)
             load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), n, wave_base, lane, val);
 
-        // Digit of slot j; keys past n get the top digit so that they rank after
-        // every real key of the tile (they are last in input order).
+        // Keys past n (last tile only) get every bit set: they take the pass's
+        // top digit (mask) and, last in input order, rank after every real key
+        // of the tile.  Digit of slot j: one bit-field extract.
         const int32_t rem = (int32_t)(n - (wave_base + lane));  // slots j*64 < rem are real
-        auto digit_of = [&](int j) -> uint32_t {
-            if (full) return (key[j] >> shift) & mask;
-            return j * 64 < rem ? ((key[j] >> shift) & mask) : (RADIX - 1);
-        };
+        if (!full) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) key[j] = j * 64 < rem ? key[j] : 0xFFFFFFFFu;
+        }
+        auto digit_of = [&](int j) -> uint32_t { return __builtin_amdgcn_ubfe(key[j], shift, width); };
 
         // Rank within the wave: pos[j] = earlier same-digit keys of this wave.
         uint32_t pos[IPT];
@@ -786,7 +791,7 @@ __device__ void rank_scatter_body(const Args& a) {
                 sum += c;
             }
             total = sum;
-            tile_count = (tid == RADIX - 1) ? sum - (TILE - tile_valid) : sum;
+            tile_count = (tid == mask) ? sum - (TILE - tile_valid) : sum;
             lookback_publish<XSEG>(&job.lookback[tile * RADIX + tid],
                                    (tile == 0 ? MGPU_LB_PREFIX : MGPU_LB_AGGREGATE) | tile_count);
         }
@@ -847,7 +852,7 @@ __device__ void rank_scatter_body(const Args& a) {
         for (int i = 0; i < IPT; ++i) kout[i] = s_stage[i * BLOCK + tid];
 #pragma unroll
         for (int i = 0; i < IPT; ++i) {
-            dest[i] = s_global[(kout[i] >> shift) & mask] + i * BLOCK + tid;
+            dest[i] = s_global[__builtin_amdgcn_ubfe(kout[i], shift, width)] + i * BLOCK + tid;
             if constexpr (JOINT) vout[i] = s_vstage[i * BLOCK + tid];
         }
         // Offsets outside [0, n) can only come from a broken count or
@@ -877,10 +882,11 @@ __device__ void rank_scatter_body(const Args& a) {
 #pragma unroll
                 for (int i = 0; i < IPT; ++i) {
                     if (dest[i] < n) {
-                        const uint32_t num = (dest[i] / TILE) * a.num_segs + a.num_segs - 1;
+                        // dest / TILE < 2^16, num_segs <= 8: a 24-bit multiply.
+                        const uint32_t num = __umul24(dest[i] / TILE, a.num_segs) + a.num_segs - 1;
                         const uint32_t xo =
                             (uint32_t)__builtin_fmaf((float)num, seg_rcp, seg_half);
-                        const uint32_t nd = (kout[i] >> a.next_shift) & a.next_mask;
+                        const uint32_t nd = __builtin_amdgcn_ubfe(kout[i], a.next_shift, next_width);
                         count1(s_next, s_spare, xo * RADIX + nd, nd == next_mode, lane);
                     }
                 }
