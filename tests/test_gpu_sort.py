@@ -110,7 +110,12 @@ def test_multi_value_arrays(engine, oracle, vc):
 @pytest.mark.parametrize("fb,eb", [(0, 1), (31, 32), (7, 25), (0, 24), (4, 12), (3, 4), (0, 9),
                                    (16, 32), (1, 31)])
 @pytest.mark.parametrize("vc", [0, 1, -1])
-def test_bit_ranges(engine, oracle, fb, eb, vc):
+@pytest.mark.parametrize("which", ["default", "xcd"])
+def test_bit_ranges(request, oracle, fb, eb, vc, which):
+    """Digit widths below 8 bits (a narrow last pass: keys past the end take
+    its top digit), with one lookback chain and with segments per XCD (nine
+    tiles: the next pass's counts use its narrower digit)."""
+    engine = request.getfixturevalue("engine" if which == "default" else "xcd_engine")
     n = 70_001
     keys = workloads.uniform_keys(n, seed=fb * 40 + eb)
     vals = [np.arange(n, dtype=np.uint32)] if vc == 1 else []
@@ -120,7 +125,9 @@ def test_bit_ranges(engine, oracle, fb, eb, vc):
 
 
 @pytest.mark.parametrize("kind", ["equal", "sorted", "reversed", "two_values", "skewed"])
-def test_adversarial_inputs(engine, oracle, kind):
+@pytest.mark.parametrize("which", ["default", "xcd"])
+def test_adversarial_inputs(request, oracle, kind, which):
+    engine = request.getfixturevalue("engine" if which == "default" else "xcd_engine")
     n = 200_000
     if kind == "equal":
         keys = np.full(n, 0xDEADBEEF, dtype=np.uint32)
