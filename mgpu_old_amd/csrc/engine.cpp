@@ -74,7 +74,7 @@ sortStatus_t ensure_lookback(SortLane& L, uint64_t words) {
 
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys) {
     const uint64_t largeTiles = (keys + e->large.tile() - 1) / e->large.tile();
-    return largeTiles >= (uint64_t)kLargeTilesPerCU * (uint64_t)e->numCUs ? e->large : e->small;
+    return largeTiles * kCUsPerLargeTile >= (uint64_t)e->numCUs ? e->large : e->small;
 }
 
 sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes) {

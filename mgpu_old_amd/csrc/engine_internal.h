@@ -68,8 +68,9 @@ struct sortEngine_d {
     hipFunction_t fnIota = nullptr;
     hipFunction_t fnMultiCount = nullptr;
     hipFunction_t fnMultiScan = nullptr;
-    // Tile shapes: `large` once a pass has at least kLargeTilesPerCU of its
-    // tiles per CU, `small` below that (more, smaller tiles keep every CU busy).
+    // Tile shapes: `large` once a pass has a large tile for every
+    // kCUsPerLargeTile CUs, `small` below that (more, smaller tiles keep the
+    // CUs busy).
     PassVariant small, large;
     uint32_t rankMethod = 0;  // 1: atomic-add ranking (checked at creation), 0: match masks
     // Segments per XCD for single-array passes: the number of XCDs when the
@@ -112,10 +113,10 @@ int mgpu_plan_passes(int numBits, int maxWidth, int* widths);
 sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, unsigned grid,
                          unsigned block, void* args);
 
-// A pass over `keys` keys in all uses the large tile shape once it has at least
-// this many tiles per CU (2^22 keys on 256 CUs), else the small one (measured:
-// 1024 x 16 is ahead from 2^22 keys up, 512 x 16 below).
-constexpr uint32_t kLargeTilesPerCU = 1;
+// A pass over `keys` keys in all uses the large tile shape once it has a
+// large tile for every this many CUs (2^21 keys on 256 CUs), else the small
+// one (measured: 1024 x 16 is ahead from 2^21 keys up, 512 x 16 below).
+constexpr uint32_t kCUsPerLargeTile = 2;
 // Segments per XCD pay a fixed cost per pass (each workgroup's counts for the
 // next pass are flushed once); they win from about this many tiles per XCD
 // (2^24 keys in 16384-key tiles on 8 XCDs).

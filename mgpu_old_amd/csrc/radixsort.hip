@@ -996,7 +996,7 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
     MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
 
 // Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest from
-// 2^22 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps);
+// 2^21 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps);
 // each with both ranking methods, with one lookback chain across the device
 // (_v) or one per XCD segment (_x_v).
 MGPU_RANK_SCATTER(256, 16, 0)

@@ -85,12 +85,12 @@ def test_sizes_and_value_counts(engine, oracle, n, vc):
     _check_vs_oracle(oracle, keys, g, vals, vc)
 
 
-@pytest.mark.parametrize("n", [255 * 16384, 255 * 16384 + 1])
+@pytest.mark.parametrize("n", [127 * 16384, 127 * 16384 + 1])
 @pytest.mark.parametrize("vc", [-1, 1])
 def test_tile_shape_switch(engine, oracle, n, vc):
-    """The engine takes the 512 x 16 tiles below 256 tiles of 1024 x 16 keys
-    (one per CU on 256 CUs) and the 1024 x 16 tiles from there on: both sides
-    of the switch against the oracle."""
+    """The engine takes the 512 x 16 tiles below 128 tiles of 1024 x 16 keys
+    (one per two CUs on 256 CUs) and the 1024 x 16 tiles from there on: both
+    sides of the switch against the oracle."""
     keys = workloads.uniform_keys(n, seed=n)
     vals = [workloads.uniform_keys(n, seed=n + 1)] if vc == 1 else []
     g = GpuSort(keys, vals, value_count=vc).run(engine)
