@@ -305,6 +305,20 @@ typedef struct {
 } sortEngineInfo_t;
 sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info);
 
+/* Signed and floating-point keys.  sortTransformKeys maps numElements int32
+ * or float32 keys in place, on the engine's stream, to unsigned keys whose
+ * order is the numeric order (toRadixOrder true), or back (false): sort int
+ * or float keys by mapping them, sortArray, and mapping the result back.
+ * int32: the sign bit flipped.  float32: non-negative values get the sign bit
+ * set, negative ones every bit flipped, so -0 sorts just below +0 and NaNs
+ * by their bits at the ends.  No sortData_d field changes; the float mapping
+ * is the reference's FloatToUint / UintToFloat (sort/src/kernels/common.cu:
+ * 159-185), which the reference defines but never calls. */
+#define SORT_KEYS_INT32 1
+#define SORT_KEYS_FLOAT32 2
+sortStatus_t SORTAPI sortTransformKeys(sortEngine_t engine, sortDevicePtr_t keys, int numElements,
+                                       int keyType, bool toRadixOrder);
+
 /* Per-kernel timing: while enabled, every launch is bracketed by hipEvents
  * on the engine's stream.  sortReadTiming waits for the stream, fills up to
  * maxEntries aggregates (one per kernel name, in first-launch order), clears

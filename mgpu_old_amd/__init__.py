@@ -47,6 +47,8 @@ class SortStatus:
 
 
 SORT_VALUE_INDEX = -1
+KEYS_INT32 = 1      # SORT_KEYS_INT32: sortTransformKeys key types
+KEYS_FLOAT32 = 2    # SORT_KEYS_FLOAT32
 SORT_MAX_ELEMENTS = (1 << 30) - 1
 
 
@@ -180,6 +182,8 @@ def lib():
             "sortServerDestroy": (ctypes.c_int, [ctypes.c_void_p]),
             "sortCheckDevice": (ctypes.c_int, [eng]),
             "sortEngineInfo": (ctypes.c_int, [eng, P(EngineInfo)]),
+            "sortTransformKeys": (ctypes.c_int, [eng, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_bool]),
             "sortEnableTiming": (ctypes.c_int, [eng, ctypes.c_bool]),
             "sortReadTiming": (ctypes.c_int, [eng, P(KernelTiming), ctypes.c_int]),
         }
@@ -252,6 +256,12 @@ class Engine:
     def check_device(self):
         """Raise if a kernel flagged an inconsistency since the last check."""
         _check(lib().sortCheckDevice(self._h), "sortCheckDevice")
+
+    def transform_keys(self, keys_ptr, n, key_type, to_radix):
+        """sortTransformKeys: int32 (KEYS_INT32) or float32 (KEYS_FLOAT32) keys
+        at a device pointer to the sort's unsigned order (to_radix) or back."""
+        _check(lib().sortTransformKeys(self._h, keys_ptr, n, key_type, bool(to_radix)),
+               "sortTransformKeys")
 
     def info(self):
         """sortEngineInfo as a dict."""

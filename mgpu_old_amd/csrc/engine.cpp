@@ -537,6 +537,23 @@ sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine) {
     return SORT_STATUS_KERNEL_ERROR;
 }
 
+sortStatus_t SORTAPI sortTransformKeys(sortEngine_t engine, sortDevicePtr_t keys, int numElements,
+                                       int keyType, bool toRadixOrder) {
+    if (!engine || numElements < 0 || numElements > SORT_MAX_ELEMENTS ||
+        (keyType != SORT_KEYS_INT32 && keyType != SORT_KEYS_FLOAT32) || (!keys && numElements))
+        return SORT_STATUS_INVALID_VALUE;
+    if (numElements == 0) return SORT_STATUS_SUCCESS;
+    mgpuKeyTransformArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.keys = reinterpret_cast<uint32_t*>(keys);
+    a.n = (uint32_t)numElements;
+    a.type = keyType == SORT_KEYS_INT32 ? MGPU_KEYS_INT32 : MGPU_KEYS_FLOAT32;
+    a.to_radix = toRadixOrder ? 1u : 0u;
+    const uint32_t grid =
+        std::min<uint32_t>(((uint32_t)numElements + 1023) / 1024, (uint32_t)engine->numCUs * 8);
+    return mgpu_launch(engine, engine->lane.stream, engine->fnKeyTransform, grid, 256, &a);
+}
+
 sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info) {
     if (!engine || !info) return SORT_STATUS_INVALID_VALUE;
     std::memset(info, 0, sizeof(*info));

@@ -66,6 +66,7 @@ struct sortEngine_d {
     hipFunction_t fnCountCols = nullptr;  // 1024 threads, lane-column histograms (large sorts)
     hipFunction_t fnScan = nullptr;
     hipFunction_t fnIota = nullptr;
+    hipFunction_t fnKeyTransform = nullptr;
     hipFunction_t fnMultiCount = nullptr;
     hipFunction_t fnMultiScan = nullptr;
     // Tile shapes: `large` once a pass has a large tile for every

@@ -116,6 +116,32 @@ struct mgpuIotaArgs {
     uint32_t pad_;
 };
 
+// key_transform: int32 or float32 keys to and from the unsigned order the
+// sort uses (in place).
+#define MGPU_KEYS_INT32 1
+#define MGPU_KEYS_FLOAT32 2
+struct mgpuKeyTransformArgs {
+    uint32_t* keys;
+    uint32_t n;
+    uint32_t type;       // MGPU_KEYS_INT32 or MGPU_KEYS_FLOAT32
+    uint32_t to_radix;   // 1: to the sort's order, 0: back
+    uint32_t pad_;
+};
+
+// The bijections of key_transform, shared with host code.  int32: flip the
+// sign bit.  float32 (the mapping of the reference's FloatToUint /
+// UintToFloat, common.cu:159-185): non-negative values get the sign bit set,
+// negative ones have every bit flipped, so unsigned order is numeric order
+// with -0 just below +0 (NaNs by their bits, at both ends).
+static inline MGPU_HOST_DEVICE uint32_t mgpu_key_to_radix(uint32_t k, uint32_t type) {
+    if (type == MGPU_KEYS_INT32) return k ^ 0x80000000u;
+    return (k & 0x80000000u) ? ~k : (k | 0x80000000u);
+}
+static inline MGPU_HOST_DEVICE uint32_t mgpu_key_from_radix(uint32_t u, uint32_t type) {
+    if (type == MGPU_KEYS_INT32) return u ^ 0x80000000u;
+    return (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+}
+
 // ---------------------------------------------------------------------------
 // Segmented sort: many independent arrays in one persistent launch.
 

@@ -135,6 +135,7 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_count_cols", &e->fnCountCols);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_key_transform", &e->fnKeyTransform);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_scan", &e->fnMultiScan);
     uint32_t smallShape[2] = {512, 16}, largeShape[2] = {1024, 16};

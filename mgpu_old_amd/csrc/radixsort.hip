@@ -968,6 +968,30 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_iota(mgpuIotaArgs a) {
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += gridDim.x * 256) a.out[i] = i;
 }
 
+// Grid-stride, 16-byte accesses where the pointer allows.
+extern "C" __global__ void __launch_bounds__(256) mgpu_key_transform(mgpuKeyTransformArgs a) {
+    const uint32_t stride = gridDim.x * 256;
+    auto f = [&](uint32_t k) {
+        return a.to_radix ? mgpu_key_to_radix(k, a.type) : mgpu_key_from_radix(k, a.type);
+    };
+    if ((reinterpret_cast<uintptr_t>(a.keys) & 15) == 0) {
+        uint4* v = reinterpret_cast<uint4*>(a.keys);
+        const uint32_t nv = a.n >> 2;
+        for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < nv; i += stride) {
+            uint4 q = v[i];
+            q.x = f(q.x);
+            q.y = f(q.y);
+            q.z = f(q.z);
+            q.w = f(q.w);
+            v[i] = q;
+        }
+        for (uint32_t i = (nv << 2) + blockIdx.x * 256 + threadIdx.x; i < a.n; i += stride)
+            a.keys[i] = f(a.keys[i]);
+    } else {
+        for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < a.n; i += stride) a.keys[i] = f(a.keys[i]);
+    }
+}
+
 extern "C" __global__ void __launch_bounds__(256) mgpu_multi_count(mgpuMultiCountArgs a) {
     multi_count_body<256>(a);
 }

@@ -28,8 +28,13 @@
 //       cpusort.cpp:144-169 (hist1/hist2/hist3).
 //   * oracle_splitmix64_*     -- the seeded input generator shared with the
 //       workload module (SplitMix64, Steele/Lea/Flood 2014; counter form).
+//   * oracle_key_transform    -- signed and floating keys to the unsigned
+//       order the sort uses and back: int32 by flipping the sign bit, float32
+//       in the steps of the reference's FloatToUint / UintToFloat
+//       (sort/src/kernels/common.cu:159-185; defined there, never called).
 //
-// Pinning: tests/test_oracle.py checks oracle_stable_sort against numpy's
+// Pinning: tests/test_oracle.py checks oracle_key_transform against numpy's
+// int32 and float32 orderings and for a round trip; it checks oracle_stable_sort against numpy's
 // stable argsort, against brute force over every permutation of small
 // inputs, against the invariants that fix a stable sort uniquely, and
 // oracle_sort1/oracle_sort2 against oracle_stable_sort; splitmix64 against
@@ -153,6 +158,30 @@ uint64_t oracle_splitmix64_at(uint64_t seed, uint64_t i) {
 //   kind 1: low-entropy keys, key[i] = word(3i) & word(3i+1) & word(3i+2)
 //           (each bit set with probability 1/8)
 // Bits at and above maskBits are cleared (maskBits 32 keeps all).
+// type 1: int32, 2: float32; toRadix 1: to the unsigned order, 0: back.
+// float32 follows common.cu:159-185 step by step: to the order, all but the
+// sign bit flipped when negative (x ^ (0x7fffffff & (x >> 31))), then 2^31
+// added; back, 2^31 subtracted and the same conditional flip.
+int oracle_key_transform(const uint32_t* in, uint64_t n, int type, int toRadix, uint32_t* out) {
+    if (type != 1 && type != 2) return -1;
+    for (uint64_t i = 0; i < n; ++i) {
+        if (type == 1) {
+            out[i] = in[i] ^ 0x80000000u;
+        } else if (toRadix) {
+            const int32_t x = (int32_t)in[i];
+            const int32_t bits = x >> 31;
+            const int32_t flipped = x ^ (0x7fffffff & bits);
+            out[i] = (uint32_t)flipped + 0x80000000u;
+        } else {
+            const uint32_t flippedU = in[i] - 0x80000000u;
+            const int32_t flipped = (int32_t)flippedU;
+            const int32_t bits = flipped >> 31;
+            out[i] = (uint32_t)(flipped ^ (0x7fffffff & bits));
+        }
+    }
+    return 0;
+}
+
 void oracle_fill_keys(uint32_t* out, uint64_t n, uint64_t seed, int kind, int maskBits) {
     uint32_t mask = maskBits >= 32 ? 0xffffffffu : ((1u << maskBits) - 1u);
     for (uint64_t i = 0; i < n; ++i) {

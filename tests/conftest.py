@@ -33,6 +33,9 @@ class Oracle:
         u32pp = P(P(ctypes.c_uint32))
         L.oracle_splitmix64_at.restype = ctypes.c_uint64
         L.oracle_splitmix64_at.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_key_transform.restype = ctypes.c_int
+        L.oracle_key_transform.argtypes = [P(ctypes.c_uint32), ctypes.c_uint64, ctypes.c_int,
+                                           ctypes.c_int, P(ctypes.c_uint32)]
         L.oracle_fill_keys.restype = None
         L.oracle_fill_keys.argtypes = [P(ctypes.c_uint32), ctypes.c_uint64, ctypes.c_uint64,
                                        ctypes.c_int, ctypes.c_int]
@@ -88,6 +91,14 @@ class Oracle:
         vc = len(values) if value_count is None else value_count
         return self._run(self.L.oracle_sort2, keys, values, vc, first_bit, end_bit, digit_bits,
                          block_size, threads)
+
+    def key_transform(self, keys, key_type, to_radix):
+        """keys: uint32 bit patterns; key_type 1 = int32, 2 = float32."""
+        keys = np.ascontiguousarray(keys, dtype=np.uint32)
+        out = np.empty_like(keys)
+        assert self.L.oracle_key_transform(_u32p(keys), keys.size, key_type, int(to_radix),
+                                           _u32p(out)) == 0
+        return out
 
     def fill_keys(self, n, seed, kind=0, mask_bits=32):
         out = np.empty(n, dtype=np.uint32)

@@ -143,6 +143,62 @@ def test_adversarial_inputs(request, oracle, kind, which):
     _check_vs_oracle(oracle, keys, g, value_count=-1)
 
 
+def _typed_keys(n, key_type, seed):
+    """int32 values, or float32 values mixing normal numbers, arbitrary bit
+    patterns (NaNs, infinities, denormals) and signed zeros, as uint32 bits."""
+    rng = np.random.default_rng(seed)
+    if key_type == mg.KEYS_INT32:
+        return rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32).view(np.uint32)
+    f = (rng.standard_normal(n) * 100).astype(np.float32)
+    pick = rng.integers(0, 4, n)
+    f = np.where(pick == 1, rng.integers(0, 2**32, n, dtype=np.uint64).astype(np.uint32)
+                 .view(np.float32), f)
+    f = np.where(pick == 2, np.float32(0.0), f)
+    f = np.where(pick == 3, np.float32(-0.0), f)
+    return f.view(np.uint32)
+
+
+@pytest.mark.parametrize("key_type", [mg.KEYS_INT32, mg.KEYS_FLOAT32])
+@pytest.mark.parametrize("n", [1, 1000, 70_001, (1 << 22) + 3])
+def test_signed_and_float_keys(engine, oracle, key_type, n):
+    """sortTransformKeys to the sort's order, sortArray with index generation,
+    sortTransformKeys back: keys and indices equal the oracle's map, stable
+    sort and map back, and the device mapping alone equals the oracle's."""
+    import torch
+    bits = _typed_keys(n, key_type, seed=n + key_type)
+    t = to_dev(bits)
+    engine.transform_keys(t.data_ptr(), n, key_type, True)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(t, n), oracle.key_transform(bits, key_type, True))
+
+    g = GpuSort(bits, value_count=-1)
+    engine.transform_keys(g.data.keys[0], n, key_type, True)
+    g.run(engine)
+    engine.transform_keys(g.data.keys[0], n, key_type, False)
+    torch.cuda.synchronize()
+    engine.check_device()
+    rk, (ri,) = oracle.stable_sort(oracle.key_transform(bits, key_type, True), value_count=-1)
+    assert np.array_equal(g.keys(), oracle.key_transform(rk, key_type, False))
+    assert np.array_equal(g.values()[0], ri)
+    if key_type == mg.KEYS_INT32:
+        assert np.array_equal(g.values()[0], np.argsort(bits.view(np.int32), kind="stable"))
+    else:
+        f = g.keys().view(np.float32)
+        f = f[~np.isnan(f)]
+        assert (np.diff(f.astype(np.float64)) >= 0).all()
+
+
+def test_transform_keys_rejects_bad_arguments(engine):
+    with pytest.raises(mg.MgpuError):
+        engine.transform_keys(0, 10, mg.KEYS_INT32, True)
+    t = to_dev(np.zeros(4, dtype=np.uint32))
+    with pytest.raises(mg.MgpuError):
+        engine.transform_keys(t.data_ptr(), 4, 7, True)
+    with pytest.raises(mg.MgpuError):
+        engine.transform_keys(t.data_ptr(), -1, mg.KEYS_FLOAT32, True)
+    engine.transform_keys(t.data_ptr(), 0, mg.KEYS_FLOAT32, True)
+
+
 def test_zero_bits_and_zero_elements_do_nothing(engine):
     keys = workloads.uniform_keys(5000, 11)
     g = GpuSort(keys, value_count=-1, first_bit=9, end_bit=9).run(engine)

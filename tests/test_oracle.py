@@ -152,3 +152,71 @@ def test_oracle_rejects_bad_ranges(oracle):
         oracle.stable_sort(keys, first_bit=5, end_bit=4)
     with pytest.raises(ValueError):
         oracle.stable_sort(keys, first_bit=0, end_bit=33)
+
+
+def _special_floats():
+    f = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.finfo(np.float32).max,
+                  -np.finfo(np.float32).max, np.finfo(np.float32).tiny, -np.finfo(np.float32).tiny,
+                  1e-45, -1e-45, 3.0, -3.0], dtype=np.float32)
+    return f.view(np.uint32)
+
+
+def test_key_transform_int32_order(oracle):
+    """Unsigned order of the mapped keys is numpy's int32 order (a stable
+    argsort by the mapped keys equals a stable argsort by the int32 values)."""
+    rng = np.random.default_rng(11)
+    v = rng.integers(-2**31, 2**31, 50000, dtype=np.int64).astype(np.int32)
+    v[:6] = [-2**31, 2**31 - 1, -1, 0, 1, -2**31]
+    u = oracle.key_transform(v.view(np.uint32), 1, True)
+    assert np.array_equal(np.argsort(u, kind="stable"), np.argsort(v, kind="stable"))
+    assert np.array_equal(oracle.key_transform(u, 1, False), v.view(np.uint32))
+
+
+def test_key_transform_float32_order(oracle):
+    """Unsigned order of the mapped floats is numeric order, with -0 below +0;
+    NaNs sort at the ends (negative-signed below -inf, positive above +inf)."""
+    rng = np.random.default_rng(12)
+    f = np.concatenate([rng.standard_normal(30000).astype(np.float32) * 1e3,
+                        rng.integers(0, 2**32, 30000, dtype=np.uint64).astype(np.uint32)
+                        .view(np.float32), _special_floats().view(np.float32)])
+    finite_or_inf = ~np.isnan(f)
+    g = f[finite_or_inf]
+    u = oracle.key_transform(g.view(np.uint32), 2, True)
+    s = g[np.argsort(u, kind="stable")]
+    assert (np.diff(s.astype(np.float64)) >= 0).all()
+    # Among zeros every -0 comes before every +0.
+    z = s[s == 0]
+    signs = np.signbit(z)
+    assert not (signs[1:] & ~signs[:-1]).any()
+    # Stable among equal values: numpy's stable order after separating -0.
+    key = g.astype(np.float64) - np.where((g == 0) & np.signbit(g), 1e-300, 0.0)
+    assert np.array_equal(np.argsort(u, kind="stable"), np.argsort(key, kind="stable"))
+    nan_pos = oracle.key_transform(np.array([0x7FC00000], dtype=np.uint32), 2, True)[0]
+    nan_neg = oracle.key_transform(np.array([0xFFC00000], dtype=np.uint32), 2, True)[0]
+    inf_pos = oracle.key_transform(np.array([0x7F800000], dtype=np.uint32), 2, True)[0]
+    inf_neg = oracle.key_transform(np.array([0xFF800000], dtype=np.uint32), 2, True)[0]
+    assert nan_neg < inf_neg < inf_pos < nan_pos
+
+
+@pytest.mark.parametrize("key_type", [1, 2])
+def test_key_transform_round_trip_and_bijection(oracle, key_type):
+    rng = np.random.default_rng(13 + key_type)
+    bits = np.concatenate([rng.integers(0, 2**32, 1 << 20, dtype=np.uint64).astype(np.uint32),
+                           np.arange(0, 1 << 16, dtype=np.uint32),
+                           np.arange(2**32 - (1 << 16), 2**32, dtype=np.uint64).astype(np.uint32),
+                           np.arange(2**31 - (1 << 15), 2**31 + (1 << 15), dtype=np.uint64)
+                           .astype(np.uint32), _special_floats()])
+    u = oracle.key_transform(bits, key_type, True)
+    assert np.array_equal(oracle.key_transform(u, key_type, False), bits)
+    assert np.unique(u).size == np.unique(bits).size
+
+
+def test_key_transform_catches_a_planted_mistake(oracle):
+    """The float order check fails for a mapping that forgets to flip the
+    magnitude of negative values (a plausible slip)."""
+    g = np.array([-2.0, -1.0, 0.5, 3.0], dtype=np.float32)
+    wrong = g.view(np.uint32) ^ np.uint32(0x80000000)
+    s = g[np.argsort(wrong, kind="stable")]
+    assert not (np.diff(s.astype(np.float64)) >= 0).all()
+    right = oracle.key_transform(g.view(np.uint32), 2, True)
+    assert (np.diff(g[np.argsort(right, kind="stable")].astype(np.float64)) >= 0).all()
