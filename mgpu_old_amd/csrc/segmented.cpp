@@ -21,10 +21,26 @@
 namespace {
 
 // Segments of at least this size go to the device-wide passes (all of them
-// together) instead of one workgroup each.
-constexpr uint32_t kDeviceWideMin = (1u << 16) + 1;
-// Keys per digit_count workgroup on that path.
-constexpr uint32_t kWideChunk = 1u << 16;
+// together, each spread over many workgroups with a lookback chain) instead
+// of one workgroup each.  Above 4096 keys (the largest a workgroup sorts in
+// LDS) one workgroup would stream the segment through global memory tile by
+// tile, bound by memory latency: a 64K-key segment took up to 1.3 ms, and a
+// batch of a few such segments left the GPU idle.  Measured: 4096 segments
+// log-uniform in [1, 2^16] in 2.04 ms with the threshold above 2^16, 0.74 ms
+// above 4096.  MGPUSORT_SEG_WIDE_MIN overrides it (measurements).
+constexpr uint32_t kDeviceWideMin = 4096 + 1;
+uint32_t device_wide_min() {
+    const char* v = std::getenv("MGPUSORT_SEG_WIDE_MIN");
+    const long x = v ? std::strtol(v, nullptr, 10) : 0;
+    return x > 0 ? (uint32_t)x : kDeviceWideMin;
+}
+// Keys per digit_count workgroup on that path (MGPUSORT_SEG_WIDE_CHUNK).
+constexpr uint32_t kWideChunk = 1u << 14;
+uint32_t wide_chunk() {
+    const char* v = std::getenv("MGPUSORT_SEG_WIDE_CHUNK");
+    const long x = v ? std::strtol(v, nullptr, 10) : 0;
+    return x >= 1024 ? (uint32_t)x & ~1023u : kWideChunk;
+}
 
 const SegTuneEntry kDefaultTable[] = {
     // max log2(n), radix bits, items per thread
@@ -135,6 +151,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     P.hostSegs.assign(segs, segs + numSegments);
 
     // Group segments by kernel variant; very large ones go device-wide.
+    const uint32_t wideMin = device_wide_min(), chunkKeys = wide_chunk();
     struct Group {
         std::vector<uint32_t> big, small;
     };
@@ -142,7 +159,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     for (int i = 0; i < numSegments; ++i) {
         const uint32_t n = segs[i].n;
         if (n == 0 || P.numBits == 0) continue;
-        if (n >= kDeviceWideMin) {
+        if (n >= wideMin) {
             P.wide.push_back((uint32_t)i);
             continue;
         }
@@ -192,10 +209,10 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             const mgpuSegDesc& d = segs[P.wide[k]];
             wideDescs.push_back(d);
             tileBegin.push_back(tileBegin.back() + (d.n + tile - 1) / tile);
-            for (uint32_t b = 0; b < d.n; b += kWideChunk) {
+            for (uint32_t b = 0; b < d.n; b += chunkKeys) {
                 chunks.push_back((uint32_t)k);
                 chunks.push_back(b);
-                chunks.push_back(std::min(d.n, b + kWideChunk));
+                chunks.push_back(std::min(d.n, b + chunkKeys));
                 chunks.push_back(0);
             }
         }

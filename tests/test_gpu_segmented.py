@@ -53,10 +53,18 @@ SIZES = [0, 1, 2, 3, 63, 64, 65, 127, 128, 129, 500, 1023, 1024, 1025, 2047, 204
 
 
 @pytest.mark.parametrize("variant", [None, "r8i16", "r8i8", "r4i8", "r5i16", "r6i8", "r7i16"])
-def test_size_classes_per_variant(engine, oracle, variant):
+@pytest.mark.parametrize("wide_min", [None, "65537"])
+def test_size_classes_per_variant(engine, oracle, variant, wide_min):
+    """Every size class with every kernel variant.  By default segments above
+    4096 keys take the device-wide passes; MGPUSORT_SEG_WIDE_MIN=65537 keeps
+    segments up to 2^16 keys in the persistent kernel, whose one-workgroup
+    global routine then sorts those above its LDS capacity."""
     old = os.environ.get("MGPUSORT_SEG_VARIANT")
+    old_wide = os.environ.get("MGPUSORT_SEG_WIDE_MIN")
     if variant:
         os.environ["MGPUSORT_SEG_VARIANT"] = variant
+    if wide_min:
+        os.environ["MGPUSORT_SEG_WIDE_MIN"] = wide_min
     try:
         starts = _segments(SIZES)
         n = int(starts[-1])
@@ -71,6 +79,10 @@ def test_size_classes_per_variant(engine, oracle, variant):
             os.environ.pop("MGPUSORT_SEG_VARIANT", None)
         else:
             os.environ["MGPUSORT_SEG_VARIANT"] = old
+        if old_wide is None:
+            os.environ.pop("MGPUSORT_SEG_WIDE_MIN", None)
+        else:
+            os.environ["MGPUSORT_SEG_WIDE_MIN"] = old_wide
 
 
 @pytest.mark.parametrize("vc", [-1, 0, 2, 6])
