@@ -202,7 +202,11 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     if (!P.wide.empty()) {
         uint64_t wideKeys = 0;
         for (uint32_t k : P.wide) wideKeys += segs[k].n;
-        P.wideVariant = &mgpu_pick_variant(e, wideKeys);
+        // Each segment's last tile is partial, so segments of a few tiles
+        // take the small tile shape (4096 segments log-uniform in [1, 2^16]:
+        // 0.74 -> 0.69 ms); the large shape once they average 4 large tiles.
+        const uint64_t avgKeys = wideKeys / P.wide.size();
+        P.wideVariant = avgKeys >= 4ull * e->large.tile() ? &mgpu_pick_variant(e, wideKeys) : &e->small;
         const uint32_t tile = P.wideVariant->tile();
         tileBegin.push_back(0);
         for (size_t k = 0; k < P.wide.size(); ++k) {
