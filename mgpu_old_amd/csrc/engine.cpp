@@ -133,6 +133,12 @@ sortStatus_t mgpu_lane_init(SortLane* l) {
 }
 
 void mgpu_lane_free(SortLane* l) {
+    if (l->side) (void)hipStreamSynchronize(l->side);
+    if (l->fork) (void)hipEventDestroy(l->fork);
+    if (l->join) (void)hipEventDestroy(l->join);
+    if (l->side) (void)hipStreamDestroy(l->side);
+    l->fork = l->join = nullptr;
+    l->side = nullptr;
     if (l->multi) (void)hipFree(l->multi);
     l->multi = nullptr;
     l->multiArrays = 0;

@@ -49,6 +49,11 @@ struct SortLane {
     uint32_t* multi = nullptr;
     uint32_t multiArrays = 0;
     bool multiDirty = false;
+    // Segmented sorts: a second stream on which the persistent kernels run
+    // beside the device-wide passes (forked from and joined back into
+    // `stream` by the two events); created on first use.
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 sortStatus_t mgpu_lane_init(SortLane* l);

@@ -317,6 +317,16 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             lane, (uint64_t)P.sortPasses * P.wideTiles * MGPU_RADIX);
         if (!st) st = mgpu_ensure_multi(lane, (uint32_t)P.wide.size());
         if (st) return st;
+        // The side stream for the persistent kernels (default priority).
+        if (!P.launches.empty() && !lane.side) {
+            if (hipStreamCreateWithFlags(&lane.side, hipStreamNonBlocking) != hipSuccess) {
+                lane.side = nullptr;
+                return SORT_STATUS_DEVICE_ERROR;
+            }
+            if (hipEventCreateWithFlags(&lane.fork, hipEventDisableTiming) != hipSuccess ||
+                hipEventCreateWithFlags(&lane.join, hipEventDisableTiming) != hipSuccess)
+                return SORT_STATUS_DEVICE_ERROR;
+        }
     }
     return SORT_STATUS_SUCCESS;
 }
@@ -401,6 +411,18 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
 // Queues the launches of a built plan on the lane's stream.
 sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
     sortEngine_d* e = P.engine;
+    // The persistent kernels and the device-wide passes sort different
+    // segments with separate scratch: with both, the former run on the
+    // lane's side stream, forked after the work already queued and joined
+    // before anything queued later (parallel branches in a captured graph).
+    const bool fork = !P.launches.empty() && !P.wide.empty() && lane.side;
+    hipStream_t ls = lane.stream;
+    if (fork) {
+        if (hipEventRecord(lane.fork, lane.stream) != hipSuccess ||
+            hipStreamWaitEvent(lane.side, lane.fork, 0) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        ls = lane.side;
+    }
     for (size_t li = 0; li < P.launches.size(); ++li) {
         const SegLaunch& L = P.launches[li];
         mgpuSegArgs a;
@@ -422,10 +444,16 @@ sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
             a.mask[p] = (1u << L.widths[p]) - 1u;
             bit += L.widths[p];
         }
-        sortStatus_t st = mgpu_launch(e, lane.stream, L.fn, L.grid, 256, &a);
+        sortStatus_t st = mgpu_launch(e, ls, L.fn, L.grid, 256, &a);
         if (st) return st;
     }
-    if (!P.wide.empty()) return mgpu_enqueue_wide(P, lane);
+    if (!P.wide.empty()) {
+        sortStatus_t st = mgpu_enqueue_wide(P, lane);
+        if (st) return st;
+    }
+    if (fork && (hipEventRecord(lane.join, lane.side) != hipSuccess ||
+                 hipStreamWaitEvent(lane.stream, lane.join, 0) != hipSuccess))
+        return SORT_STATUS_DEVICE_ERROR;
     return SORT_STATUS_SUCCESS;
 }
 
