@@ -5,7 +5,8 @@
 // rank_scatter launch per digit pass.
 //
 // Reference kernels this replaces (seanbaxter/mgpu_old, sort/src/kernels):
-//   CountBuckets_n            count.cu:48-333      -> mgpu_digit_count
+//   CountBuckets_n            count.cu:48-333      -> mgpu_digit_count,
+//                                                  mgpu_digit_count_cols
 //   HistogramReduce_{1,2,3}_n hist1.cu, hist2.cu,  -> mgpu_digit_scan plus the
 //                             hist3.cu             decoupled lookback inside
 //                                                  rank_scatter

@@ -138,6 +138,58 @@ void pick_variant(const sortEngine_d* e, uint32_t n, int* rb, int* ipt) {
 
 // Builds the launches of one segmented sort.  Device-side arrays are owned by
 // the plan; nothing is launched here.
+sortStatus_t mgpu_lane_ensure_side(SortLane& lane) {
+    if (lane.side) return SORT_STATUS_SUCCESS;
+    // Default priority, as the lane's own stream.
+    if (hipStreamCreateWithFlags(&lane.side, hipStreamNonBlocking) != hipSuccess) {
+        lane.side = nullptr;
+        return SORT_STATUS_DEVICE_ERROR;
+    }
+    if (hipEventCreateWithFlags(&lane.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&lane.join, hipEventDisableTiming) != hipSuccess)
+        return SORT_STATUS_DEVICE_ERROR;
+    return SORT_STATUS_SUCCESS;
+}
+
+size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys) {
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    const size_t segs = std::max<size_t>(1, numSegments);
+    const size_t chunks = keys / wide_chunk() + segs;
+    return 2 * al(sizeof(mgpuSegDesc) * segs) + al(sizeof(uint32_t) * segs) +
+           al(sizeof(uint32_t) * (segs + 1)) + al(4 * sizeof(uint32_t) * chunks);
+}
+
+sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterWords,
+                                hipStream_t stream) {
+    // Layout: [payload capacity = hostBytes][counterWords counters].  The
+    // counters are only ever left zero by the kernels, so they are cleared
+    // once, here.
+    if (arena->hostBytes >= payload && arena->counterWords >= counterWords)
+        return SORT_STATUS_SUCCESS;
+    if (hipStreamSynchronize(stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    mgpu_arena_free(arena);
+    const size_t payloadCap = 2 * payload + 4096;
+    const size_t words = 2 * std::max<size_t>(counterWords, 64);
+    const size_t cap = payloadCap + words * sizeof(uint32_t);
+    if (hipMalloc(&arena->dev, cap) != hipSuccess) {
+        arena->dev = nullptr;
+        return SORT_STATUS_DEVICE_ALLOC_FAILED;
+    }
+    if (hipHostMalloc(&arena->host, payloadCap, hipHostMallocDefault) != hipSuccess) {
+        arena->host = nullptr;
+        mgpu_arena_free(arena);
+        return SORT_STATUS_HOST_ALLOC_FAILED;
+    }
+    arena->devBytes = cap;
+    arena->hostBytes = payloadCap;
+    arena->counterWords = words;
+    // On the stream, ahead of the descriptor copies that follow: a
+    // null-stream fill could land after such a copy on a non-blocking stream
+    // and zero the descriptors the kernels then follow.
+    if (hipMemsetAsync(arena->dev, 0, cap, stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+    return SORT_STATUS_SUCCESS;
+}
+
 sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
                             int numSegments, int valueCount, int firstBit, int endBit,
                             SegPlanData* out, SegArena* arena) {
@@ -262,33 +314,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
     } else {
-        // Counters live at the end of the arena and are only ever left zero
-        // by the kernels, so they are cleared once, when the arena (re)grows.
-        // Layout: [payload capacity = hostBytes][counterWords counters].
-        if (arena->hostBytes < payload || arena->counterWords < ctrWords) {
-            if (hipStreamSynchronize(lane.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
-            mgpu_arena_free(arena);
-            const size_t payloadCap = 2 * payload + 4096;
-            const size_t words = 2 * std::max<size_t>(ctrWords, 64);
-            const size_t cap = payloadCap + words * sizeof(uint32_t);
-            if (hipMalloc(&arena->dev, cap) != hipSuccess) {
-                arena->dev = nullptr;
-                return SORT_STATUS_DEVICE_ALLOC_FAILED;
-            }
-            if (hipHostMalloc(&arena->host, payloadCap, hipHostMallocDefault) != hipSuccess) {
-                arena->host = nullptr;
-                mgpu_arena_free(arena);
-                return SORT_STATUS_HOST_ALLOC_FAILED;
-            }
-            arena->devBytes = cap;
-            arena->hostBytes = payloadCap;
-            arena->counterWords = words;
-            // On the lane's stream, ahead of the descriptor copy below: a
-            // null-stream fill could land after that copy on a non-blocking
-            // lane stream and zero the descriptors the kernels then follow.
-            if (hipMemsetAsync(arena->dev, 0, cap, lane.stream) != hipSuccess)
-                return SORT_STATUS_DEVICE_ERROR;
-        }
+        // Queue counters at the end of the arena, zero between builds.
+        sortStatus_t st = mgpu_arena_reserve(arena, payload, ctrWords, lane.stream);
+        if (st) return st;
         base = static_cast<char*>(arena->dev);
         char* h = static_cast<char*>(arena->host);
         if (numSegments) std::memcpy(h, segs, sizeof(mgpuSegDesc) * numSegments);
@@ -317,16 +345,8 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             lane, (uint64_t)P.sortPasses * P.wideTiles * MGPU_RADIX);
         if (!st) st = mgpu_ensure_multi(lane, (uint32_t)P.wide.size());
         if (st) return st;
-        // The side stream for the persistent kernels (default priority).
-        if (!P.launches.empty() && !lane.side) {
-            if (hipStreamCreateWithFlags(&lane.side, hipStreamNonBlocking) != hipSuccess) {
-                lane.side = nullptr;
-                return SORT_STATUS_DEVICE_ERROR;
-            }
-            if (hipEventCreateWithFlags(&lane.fork, hipEventDisableTiming) != hipSuccess ||
-                hipEventCreateWithFlags(&lane.join, hipEventDisableTiming) != hipSuccess)
-                return SORT_STATUS_DEVICE_ERROR;
-        }
+        if (!P.launches.empty()) st = mgpu_lane_ensure_side(lane);
+        if (st) return st;
     }
     return SORT_STATUS_SUCCESS;
 }

@@ -88,5 +88,17 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
                             int numSegments, int valueCount, int firstBit, int endBit,
                             SegPlanData* out, SegArena* arena = nullptr);
 void mgpu_arena_free(SegArena* a);
+// Grows an arena (synchronizing `stream` first, whose work may still read the
+// old one) to hold `payload` bytes of plan arrays and `counterWords` queue
+// counters; does nothing when it already does.
+sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterWords,
+                                hipStream_t stream);
+// Upper bound on the plan arrays of a build over `numSegments` segments
+// holding `keys` keys in total, and on its queue counters.
+size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys);
+constexpr size_t kSegCounterWords = 4 * kSegVariants;
+// The lane's side stream and fork/join events, on which the persistent
+// kernels of a plan run beside its device-wide passes.
+sortStatus_t mgpu_lane_ensure_side(SortLane& lane);
 sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane);
 void mgpu_seg_release(SegPlanData& P);

@@ -36,6 +36,15 @@ using Clock = std::chrono::steady_clock;
 constexpr size_t kAlign = 256;
 // Smallest batch worth a slot of its own when admitted work is split.
 constexpr uint64_t kMinBatchKeys = 1ull << 22;
+// A batch holds at most kMaxBatchRequests requests and, unless it is one
+// request, at most kMaxBatchKeys keys.  Each slot's scratch (lookback words,
+// per-segment histograms, the plan arrays of kReservedClasses classes) is
+// sized for such a batch when the server starts: growing it mid-stream meant
+// hipFree / hipHostMalloc calls that waited for the whole device, and batch
+// launches 2-9 ms apart while every slot stalled.
+constexpr uint32_t kMaxBatchRequests = 1024;
+constexpr uint64_t kMaxBatchKeys = 1ull << 26;
+constexpr size_t kReservedClasses = 12;
 
 inline size_t align_up(size_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
 
@@ -277,7 +286,8 @@ void worker_main(sortServer_d* s) {
         // Launch what is admitted, spread over the free slots so that large
         // requests on different slots run side by side: each free slot takes
         // requests in arrival order up to an even share of the admitted keys
-        // (at least kMinBatchKeys, and always at least one request).
+        // (at least kMinBatchKeys, at most the batch limits, and always at
+        // least one request).
         if (!s->admitted.empty()) {
             size_t freeSlots = 0;
             for (Slot& slot : s->slots) freeSlots += !slot.busy;
@@ -290,8 +300,12 @@ void worker_main(sortServer_d* s) {
                 if (slot.busy || next >= s->admitted.size()) continue;
                 std::vector<Pending> batch;
                 uint64_t taken = 0;
-                while (next < s->admitted.size() && (batch.empty() || taken < share)) {
-                    taken += (uint64_t)s->admitted[next].req.numElements;
+                while (next < s->admitted.size()) {
+                    const uint64_t n = (uint64_t)s->admitted[next].req.numElements;
+                    if (!batch.empty() && (taken >= share || batch.size() >= kMaxBatchRequests ||
+                                           taken + n > kMaxBatchKeys))
+                        break;
+                    taken += n;
                     batch.push_back(s->admitted[next++]);
                 }
                 sortStatus_t st = launch_batch(s, slot, batch);
@@ -341,9 +355,20 @@ sortStatus_t SORTAPI sortServerCreate(sortEngine_t engine, unsigned long long hb
             break;
         }
         st = mgpu_lane_init(&slot.lane);
-        // Size the device-wide scratch for the largest request up front, so
-        // that no batch reallocates it.
-        if (!st) st = mgpu_ensure_lookback(engine, slot.lane, (uint32_t)maxElements, MGPU_MAX_PASSES);
+        // Scratch for the largest batch, so that no batch grows it: lookback
+        // words for every tile of its device-wide segments (each at most one
+        // partial tile past its keys), a histogram per segment, plan arrays.
+        const uint64_t batchKeys = std::max<uint64_t>(kMaxBatchKeys, (uint64_t)maxElements);
+        const uint64_t tile = std::min(engine->small.tile(), engine->large.tile());
+        const uint64_t tiles = batchKeys / tile + kMaxBatchRequests + 1;
+        if (!st) st = mgpu_ensure_lookback_words(slot.lane, (uint64_t)MGPU_MAX_PASSES * tiles * MGPU_RADIX);
+        if (!st) st = mgpu_ensure_multi(slot.lane, kMaxBatchRequests);
+        if (!st) st = mgpu_lane_ensure_side(slot.lane);
+        slot.arenas.resize(kReservedClasses);
+        const size_t payload = mgpu_seg_payload_bound(kMaxBatchRequests, batchKeys);
+        for (SegArena& a : slot.arenas)
+            if (!st) st = mgpu_arena_reserve(&a, payload, kSegCounterWords, slot.lane.stream);
+        if (!st && hipStreamSynchronize(slot.lane.stream) != hipSuccess) st = SORT_STATUS_DEVICE_ERROR;
     }
     if (st) {
         sortServerDestroy(s);
