@@ -303,7 +303,11 @@ def served_bench(torch, eng, count):
                 "batches": len({c.batch for c in comps}), "makespan_ms": span}
 
     run(np.zeros(count))  # warm-up
-    cap = run(np.zeros(count))  # all at t=0: capacity
+    # Capacity: every request at t=0; the median of three such runs (one run
+    # spans ~20 ms, so one slow batch moves it by several Gkeys/s).
+    bursts = sorted((run(np.zeros(count)) for _ in range(3)), key=lambda r: r["gkeys_per_s"])
+    cap = bursts[1]
+    cap["gkeys_per_s_runs"] = [b["gkeys_per_s"] for b in bursts]
     capacity = cap["gkeys_per_s"] * 1e9
     mean_keys = total / count
     out = {"requests": count, "keys": total, "hbm_budget_bytes": budget, "max_in_flight": 3,

@@ -54,6 +54,7 @@ struct SortLane {
     // `stream` by the two events); created on first use.
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    bool noSide = false;  // never fork: everything on `stream` (sort server slots)
 };
 
 sortStatus_t mgpu_lane_init(SortLane* l);

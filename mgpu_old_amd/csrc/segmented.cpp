@@ -345,7 +345,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             lane, (uint64_t)P.sortPasses * P.wideTiles * MGPU_RADIX);
         if (!st) st = mgpu_ensure_multi(lane, (uint32_t)P.wide.size());
         if (st) return st;
-        if (!P.launches.empty()) st = mgpu_lane_ensure_side(lane);
+        if (!P.launches.empty() && !lane.noSide) st = mgpu_lane_ensure_side(lane);
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;

@@ -363,7 +363,12 @@ sortStatus_t SORTAPI sortServerCreate(sortEngine_t engine, unsigned long long hb
         const uint64_t tiles = batchKeys / tile + kMaxBatchRequests + 1;
         if (!st) st = mgpu_ensure_lookback_words(slot.lane, (uint64_t)MGPU_MAX_PASSES * tiles * MGPU_RADIX);
         if (!st) st = mgpu_ensure_multi(slot.lane, kMaxBatchRequests);
-        if (!st) st = mgpu_lane_ensure_side(slot.lane);
+        // Each slot keeps its batch on its one stream: the side stream a
+        // segmented sort forks its persistent kernels onto (a gain for one
+        // sort alone) cost the server a third of its throughput, with batches
+        // of up to 9 classes on 3 slots (requests all at once: 44-51 Gkeys/s
+        // with it, 62-64 without, at 4 or 8 hardware queues alike).
+        slot.lane.noSide = true;
         slot.arenas.resize(kReservedClasses);
         const size_t payload = mgpu_seg_payload_bound(kMaxBatchRequests, batchKeys);
         for (SegArena& a : slot.arenas)
