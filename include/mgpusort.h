@@ -266,7 +266,11 @@ struct sortServer_d;
 typedef struct sortServer_d* sortServer_t;
 
 /* hbmBudgetBytes: scratch slab for admitted requests.  maxInFlight: batches
- * on the device at once (1..16).  maxElements: largest request accepted. */
+ * on the device at once (1..16).  maxElements: largest request accepted.
+ * Besides the slab, each in-flight slot holds the scratch of its largest
+ * batch (1024 requests, 2^26 keys or one request of maxElements): about
+ * 65 MB of device memory and 8 MB of pinned host memory at the default
+ * tile shapes, allocated here so that no batch allocates. */
 sortStatus_t SORTAPI sortServerCreate(sortEngine_t engine, unsigned long long hbmBudgetBytes,
                                       int maxInFlight, int maxElements, sortServer_t* server);
 /* Thread-safe.  The buffers must stay untouched until the completion. */
