@@ -115,3 +115,63 @@ def test_submit_and_poll(engine, oracle):
         keys, vals, w = host[i]
         rk, _ = oracle.stable_sort(keys, vals, first_bit=0, end_bit=w)
         assert np.array_equal(to_host(tensors[i][0], keys.size), rk)
+
+
+def test_batch_limits(engine, oracle):
+    """A batch holds at most 1024 requests and, unless it is one request,
+    at most 2^26 keys; requests past either limit run in later batches with
+    the same results."""
+    import torch
+    # 1100 tiny requests at once on one slot: at least two batches.
+    sizes = workloads.log_uniform_sizes(1100, 0, 6, seed=41)
+    reqs, host, tensors = [], [], []
+    for i, n in enumerate(sizes):
+        keys = workloads.uniform_keys(int(n), seed=4100 + i)
+        vals = [workloads.uniform_keys(int(n), seed=9100 + i)]
+        tk, tv = to_dev(keys), to_dev(vals[0])
+        r = mg.SortRequest()
+        r.keys, r.values[0] = tk.data_ptr(), tv.data_ptr()
+        r.numElements, r.valueCount, r.firstBit, r.endBit = int(n), 1, 0, 32
+        reqs.append(r)
+        host.append((keys, vals))
+        tensors.append((tk, tv))
+    torch.cuda.synchronize()
+    with mg.Server(engine, 1 << 26, max_in_flight=1, max_elements=1 << 10) as srv:
+        comps = srv.replay(reqs, np.zeros(len(reqs)))
+    torch.cuda.synchronize()
+    per_batch = {}
+    for i, c in enumerate(comps):
+        assert c.status == 0, f"request {i} failed: {mg.status_string(c.status)}"
+        per_batch[c.batch] = per_batch.get(c.batch, 0) + 1
+        keys, vals = host[i]
+        rk, rv = oracle.stable_sort(keys, vals)
+        assert np.array_equal(to_host(tensors[i][0], keys.size), rk), f"request {i} keys"
+        assert np.array_equal(to_host(tensors[i][1], keys.size), rv[0]), f"request {i} values"
+    assert len(per_batch) >= 2 and max(per_batch.values()) <= 1024
+
+    # Five keys-only requests just under 2^24 keys (80M in all) at once on
+    # one slot: no batch of several requests passes 2^26 keys.
+    n = (1 << 24) - 1000
+    reqs, host, tensors = [], [], []
+    for i in range(5):
+        keys = workloads.uniform_keys(n, seed=5100 + i)
+        tk = to_dev(keys)
+        r = mg.SortRequest()
+        r.keys = tk.data_ptr()
+        r.numElements, r.valueCount, r.firstBit, r.endBit = n, 0, 0, 32
+        reqs.append(r)
+        host.append(keys)
+        tensors.append(tk)
+    torch.cuda.synchronize()
+    with mg.Server(engine, 1 << 30, max_in_flight=1, max_elements=1 << 24) as srv:
+        comps = srv.replay(reqs, np.zeros(len(reqs)))
+    torch.cuda.synchronize()
+    keys_per_batch, reqs_per_batch = {}, {}
+    for i, c in enumerate(comps):
+        assert c.status == 0, f"request {i} failed: {mg.status_string(c.status)}"
+        keys_per_batch[c.batch] = keys_per_batch.get(c.batch, 0) + n
+        reqs_per_batch[c.batch] = reqs_per_batch.get(c.batch, 0) + 1
+        # Keys alone: the stable sort is the plain sorted order.
+        assert np.array_equal(to_host(tensors[i], n), np.sort(host[i])), f"request {i} keys"
+    for b, k in keys_per_batch.items():
+        assert reqs_per_batch[b] == 1 or k <= 1 << 26
