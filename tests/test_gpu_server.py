@@ -175,3 +175,45 @@ def test_batch_limits(engine, oracle):
         assert np.array_equal(to_host(tensors[i], n), np.sort(host[i])), f"request {i} keys"
     for b, k in keys_per_batch.items():
         assert reqs_per_batch[b] == 1 or k <= 1 << 26
+
+
+def test_submit_from_several_threads(engine, oracle):
+    """sortServerSubmit from four threads at once (ctypes drops the GIL in the
+    call) while the main thread polls: every request completes once, sorted."""
+    import threading
+    import time
+
+    import torch
+    reqs, host, tensors = _make_requests(64, seed=51, lo_log2=8, hi_log2=17)
+    torch.cuda.synchronize()
+    errors = []
+    with mg.Server(engine, 1 << 30, max_in_flight=3, max_elements=1 << 17) as srv:
+        def submit(part):
+            try:
+                for i in range(part, len(reqs), 4):
+                    reqs[i].tag = 1000 + i
+                    srv.submit(reqs[i])
+            except Exception as e:  # reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=submit, args=(p,)) for p in range(4)]
+        for t in threads:
+            t.start()
+        got = {}
+        deadline = time.monotonic() + 60
+        while len(got) < len(reqs) and not errors and time.monotonic() < deadline:
+            for c in srv.poll(timeout_ms=100):
+                assert c.tag not in got, f"request {c.tag} completed twice"
+                got[c.tag] = c
+        for t in threads:
+            t.join()
+    assert not errors, errors
+    assert len(got) == len(reqs), f"{len(reqs) - len(got)} requests never completed"
+    torch.cuda.synchronize()
+    for i in range(len(reqs)):
+        assert got[1000 + i].status == 0
+        keys, vals, w = host[i]
+        rk, rv = oracle.stable_sort(keys, vals, first_bit=0, end_bit=w)
+        assert np.array_equal(to_host(tensors[i][0], keys.size), rk), f"request {i} keys"
+        for j, t in enumerate(tensors[i][1]):
+            assert np.array_equal(to_host(t, keys.size), rv[j]), f"request {i} values[{j}]"
