@@ -19,7 +19,9 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -75,6 +77,60 @@ sortStatus_t ensure_lookback(SortLane& L, uint64_t words) {
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys) {
     const uint64_t largeTiles = (keys + e->large.tile() - 1) / e->large.tile();
     return largeTiles * kCUsPerLargeTile >= (uint64_t)e->numCUs ? e->large : e->small;
+}
+
+SortPlanChoice mgpu_pick_plan(const sortEngine_d* e, uint32_t n, int valueCount) {
+    const int cls = valueCount == 0    ? MGPU_SORT_CLASS_KEYS
+                    : valueCount == -1 ? MGPU_SORT_CLASS_INDEX
+                    : valueCount == 1  ? MGPU_SORT_CLASS_KV
+                                       : MGPU_SORT_CLASS_MULTI;
+    int lg = 0;
+    while (lg < 31 && (1u << lg) < n) ++lg;
+    const sortEngine_d::SortTuneEntry* entry = nullptr;
+    for (const auto& en : e->sortTable[cls])
+        if (lg <= en.maxLog2) {
+            entry = &en;
+            break;
+        }
+    SortPlanChoice c;
+    c.variant = entry ? (entry->large ? &e->large : &e->small) : &mgpu_pick_variant(e, n);
+    const uint32_t numTiles = (n + c.variant->tile() - 1) / c.variant->tile();
+    // Segments need a tile per XCD; from kXsegMinTilesPerSeg tiles per XCD
+    // they win (measured), unless the table or MGPUSORT_XCD=1 says otherwise.
+    bool xcd;
+    if (e->xcdForce) xcd = true;
+    else if (entry) xcd = entry->xcd;
+    else xcd = numTiles >= kXsegMinTilesPerSeg * e->xcdSegs;
+    c.segs = e->xcdSegs > 1 && xcd && numTiles >= e->xcdSegs ? e->xcdSegs : 1u;
+    return c;
+}
+
+void mgpu_load_sort_table(sortEngine_d* e) {
+    for (auto& t : e->sortTable) t.clear();
+    std::ifstream f(e->kernelPath + "mgpusort_sort_tuning.txt");
+    std::string line;
+    while (f && std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream is(line);
+        int maxLog2 = 0;
+        std::string values, tile, chains;
+        if (!(is >> maxLog2 >> values >> tile >> chains)) continue;
+        const int cls = values == "keys" ? MGPU_SORT_CLASS_KEYS
+                        : values == "index" ? MGPU_SORT_CLASS_INDEX
+                        : values == "kv" ? MGPU_SORT_CLASS_KV
+                        : values == "multi" ? MGPU_SORT_CLASS_MULTI
+                                            : -1;
+        if (cls < 0 || maxLog2 < 0 || (tile != "small" && tile != "large") ||
+            (chains != "one" && chains != "xcd"))
+            continue;
+        e->sortTable[cls].push_back({maxLog2, tile == "large", chains == "xcd"});
+    }
+    for (auto& t : e->sortTable) {
+        std::stable_sort(t.begin(), t.end(),
+                         [](const sortEngine_d::SortTuneEntry& a,
+                            const sortEngine_d::SortTuneEntry& b) { return a.maxLog2 < b.maxLog2; });
+        if (!t.empty()) t.back().maxLog2 = 31;
+    }
 }
 
 sortStatus_t mgpu_ensure_lookback(sortEngine_d* e, SortLane& lane, uint32_t n, int passes) {
@@ -210,7 +266,8 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     if (executed) *executed = 0;
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     if (numPasses > MGPU_MAX_PASSES) return SORT_STATUS_INTERNAL_ERROR;
-    const PassVariant& var = mgpu_pick_variant(e, n);
+    const SortPlanChoice choice = mgpu_pick_plan(e, n, valueCount);
+    const PassVariant& var = *choice.variant;
     const uint32_t tile = var.tile();
     const uint32_t numTiles = (n + tile - 1) / tile;
     const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
@@ -220,8 +277,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     // large sorts; else one chain across the device.  digit_count counts the
     // segments of the input order, which only the first executed pass reads;
     // each pass counts the next one's segments as it writes.
-    const uint32_t minTiles = e->xcdForce ? e->xcdSegs : kXsegMinTilesPerSeg * e->xcdSegs;
-    const uint32_t segs = e->xcdSegs > 1 && numTiles >= minTiles ? e->xcdSegs : 1u;
+    const uint32_t segs = choice.segs;
 
     if (L.histDirty) {
         if (hipMemsetAsync(L.hist, 0, kLaneHistWords * sizeof(uint32_t), L.stream) !=

@@ -90,6 +90,16 @@ struct sortEngine_d {
     // Set by kernels that detect an inconsistency (sortCheckDevice).
     uint32_t* errorFlag = nullptr;
 
+    // sortArray tuning table (kernelPath + "mgpusort_sort_tuning.txt", written
+    // by the autotuner): per value class (MGPU_SORT_CLASS_*), entries sorted
+    // by maxLog2; a class without entries uses the built-in rules.
+    struct SortTuneEntry {
+        int maxLog2;
+        bool large;  // tile shape: large (else small)
+        bool xcd;    // a lookback chain per XCD (else one across the device)
+    };
+    std::vector<SortTuneEntry> sortTable[4];
+
     // Segmented sort: kernel variants [radix bits - 4][items 16?] and the
     // size-bucket tuning table.
     hipModule_t segModule = nullptr;
@@ -132,6 +142,21 @@ constexpr uint32_t kXsegMinTilesPerSeg = 128;
 // from this many keys up.
 constexpr uint32_t kCountColsMinKeys = 1u << 22;
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys);
+
+// Value classes of the sortArray tuning table.
+enum { MGPU_SORT_CLASS_KEYS = 0, MGPU_SORT_CLASS_INDEX = 1, MGPU_SORT_CLASS_KV = 2,
+       MGPU_SORT_CLASS_MULTI = 3 };
+// Tile shape and lookback segments (1, or one per XCD) of a single-array sort
+// of n keys: the tuning table's entry for its size and value class, else the
+// built-in rules; MGPUSORT_TILE and MGPUSORT_XCD override both.
+struct SortPlanChoice {
+    const PassVariant* variant;
+    uint32_t segs;
+};
+SortPlanChoice mgpu_pick_plan(const sortEngine_d* e, uint32_t n, int valueCount);
+// Reads kernelPath + "mgpusort_sort_tuning.txt" into e->sortTable (missing
+// file or no valid line: every class keeps the built-in rules).
+void mgpu_load_sort_table(sortEngine_d* e);
 
 // Grows the lane's lookback words to cover a device-wide sort of n keys in
 // `passes` passes (a reallocation synchronizes the lane's stream).

@@ -158,6 +158,7 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     if (r && (*r == '0' || *r == '1') && r[1] == 0) e->rankMethod = (uint32_t)(*r - '0');
     else e->rankMethod = lds_add_in_lane_order(e) ? 1u : 0u;
     e->xcdSegs = xcd_segments(e);
+    mgpu_load_sort_table(e);
     st = load_variant(e, smallShape[0], smallShape[1], e->rankMethod, &e->small);
     if (!st) st = load_variant(e, largeShape[0], largeShape[1], e->rankMethod, &e->large);
     if (st) return st;
