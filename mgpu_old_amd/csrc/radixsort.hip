@@ -365,7 +365,9 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
     for (uint32_t i = begin + tid; i < vbeg; i += BLOCK) count_key(keys[i]);
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     const u32x4* v = reinterpret_cast<const u32x4*>(keys + vbeg);
-    constexpr int UNROLL = 4;
+    // A 16K-key chunk is 16 vectors per thread: all of them in flight at once
+    // (4096 segments in [1, 2^16]: 78 -> 73 us per launch).
+    constexpr int UNROLL = 16;
     for (uint32_t i0 = tid; i0 < nvec; i0 += UNROLL * BLOCK) {
         u32x4 q[UNROLL];
 #pragma unroll
