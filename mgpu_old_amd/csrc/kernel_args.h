@@ -206,6 +206,7 @@ struct mgpuMultiScanArgs {
 struct mgpuMultiPassArgs {
     const mgpuSegDesc* segs;
     const uint32_t* tile_begin;   // [num_segs + 1]: first global tile of each array
+    const uint32_t* tile_seg;     // [num_tiles]: the array of each global tile
     const uint32_t* digit_start;  // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX]
     uint32_t* lookback;           // [num_tiles][MGPU_RADIX] for this pass, zero on entry
     uint32_t* tile_counter;       // zero on entry

@@ -513,16 +513,12 @@ struct TileJob {
 };
 
 // Global tile t of a pass: which array (MULTI) and where its data lives.
+// One load of the host-built tile map: a binary search over tile_begin was a
+// chain of ~10 dependent global loads ahead of every tile.
 template <bool MULTI, typename Args>
 __device__ __forceinline__ uint32_t find_array(const Args& a, uint32_t t) {
     if constexpr (MULTI) {
-        uint32_t lo = 0, hi = a.num_segs;  // tile_begin[lo] <= t < tile_begin[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (a.tile_begin[mid] <= t) lo = mid;
-            else hi = mid;
-        }
-        return lo;
+        return a.tile_seg[t];
     } else {
         (void)a;
         (void)t;

@@ -151,12 +151,14 @@ sortStatus_t mgpu_lane_ensure_side(SortLane& lane) {
     return SORT_STATUS_SUCCESS;
 }
 
-size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys) {
+size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys, uint32_t minTile) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t segs = std::max<size_t>(1, numSegments);
     const size_t chunks = keys / wide_chunk() + segs;
+    const size_t tiles = keys / std::max(1u, minTile) + segs;
     return 2 * al(sizeof(mgpuSegDesc) * segs) + al(sizeof(uint32_t) * segs) +
-           al(sizeof(uint32_t) * (segs + 1)) + al(4 * sizeof(uint32_t) * chunks);
+           al(sizeof(uint32_t) * (segs + 1)) + al(4 * sizeof(uint32_t) * chunks) +
+           al(sizeof(uint32_t) * tiles);
 }
 
 sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterWords,
@@ -250,7 +252,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     // Device-wide segments: their descriptors, first tile of each, and the
     // digit_count chunks.
     std::vector<mgpuSegDesc> wideDescs;
-    std::vector<uint32_t> tileBegin, chunks;
+    std::vector<uint32_t> tileBegin, chunks, tileSeg;
     if (!P.wide.empty()) {
         uint64_t wideKeys = 0;
         for (uint32_t k : P.wide) wideKeys += segs[k].n;
@@ -265,6 +267,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             const mgpuSegDesc& d = segs[P.wide[k]];
             wideDescs.push_back(d);
             tileBegin.push_back(tileBegin.back() + (d.n + tile - 1) / tile);
+            tileSeg.resize(tileBegin.back(), (uint32_t)k);
             for (uint32_t b = 0; b < d.n; b += chunkKeys) {
                 chunks.push_back((uint32_t)k);
                 chunks.push_back(b);
@@ -281,8 +284,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     const size_t wideBytes = al(sizeof(mgpuSegDesc) * wideDescs.size());
     const size_t tbBytes = al(sizeof(uint32_t) * tileBegin.size());
     const size_t chBytes = al(sizeof(uint32_t) * chunks.size());
+    const size_t tsBytes = al(sizeof(uint32_t) * tileSeg.size());
     const size_t ctrWords = 4 * std::max<size_t>(1, P.launches.size());
-    const size_t payload = descBytes + listBytes + wideBytes + tbBytes + chBytes;
+    const size_t payload = descBytes + listBytes + wideBytes + tbBytes + chBytes + tsBytes;
     const size_t total = payload + ctrWords * sizeof(uint32_t);
     char* base = nullptr;
     if (!arena) {
@@ -305,7 +309,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
              hipMemcpy(base + off + wideBytes, tileBegin.data(), sizeof(uint32_t) * tileBegin.size(),
                        hipMemcpyHostToDevice) != hipSuccess ||
              hipMemcpy(base + off + wideBytes + tbBytes, chunks.data(),
-                       sizeof(uint32_t) * chunks.size(), hipMemcpyHostToDevice) != hipSuccess))
+                       sizeof(uint32_t) * chunks.size(), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(base + off + wideBytes + tbBytes + chBytes, tileSeg.data(),
+                       sizeof(uint32_t) * tileSeg.size(), hipMemcpyHostToDevice) != hipSuccess))
             return SORT_STATUS_DEVICE_ERROR;
         // Stream-ordered before the launches on the lane's stream (a plain
         // hipMemset would run on the null stream, unordered with a
@@ -326,6 +332,8 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             std::memcpy(h + off, wideDescs.data(), sizeof(mgpuSegDesc) * wideDescs.size());
             std::memcpy(h + off + wideBytes, tileBegin.data(), sizeof(uint32_t) * tileBegin.size());
             std::memcpy(h + off + wideBytes + tbBytes, chunks.data(), sizeof(uint32_t) * chunks.size());
+            std::memcpy(h + off + wideBytes + tbBytes + chBytes, tileSeg.data(),
+                        sizeof(uint32_t) * tileSeg.size());
         }
         if (hipMemcpyAsync(base, h, payload, hipMemcpyHostToDevice, lane.stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
@@ -337,6 +345,8 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     P.devWide = reinterpret_cast<mgpuSegDesc*>(base + descBytes + listBytes);
     P.devTileBegin = reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes);
     P.devChunks = reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes + tbBytes);
+    P.devTileSeg =
+        reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes + tbBytes + chBytes);
 
     // Device-wide segments: lookback words and per-array scratch before any
     // capture.
@@ -403,6 +413,7 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         std::memset(&pa, 0, sizeof(pa));
         pa.segs = P.devWide;
         pa.tile_begin = P.devTileBegin;
+        pa.tile_seg = P.devTileSeg;
         pa.digit_start = digitStart;
         pa.lookback = lane.lookback + (uint64_t)p * P.wideTiles * MGPU_RADIX;
         pa.tile_counter = lane.tileCounters + MGPU_TILE_COUNTER(p, 0);

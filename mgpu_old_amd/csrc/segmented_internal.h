@@ -69,6 +69,7 @@ struct SegPlanData {
     uint32_t wideChunks = 0;     // digit_count chunks of them
     mgpuSegDesc* devWide = nullptr;
     uint32_t* devTileBegin = nullptr;
+    uint32_t* devTileSeg = nullptr;  // [wideTiles]: the device-wide segment of each tile
     uint32_t* devChunks = nullptr;
     void* devBlock = nullptr;  // owned allocation (null when an arena holds the arrays)
     mgpuSegDesc* devSegs = nullptr;
@@ -94,8 +95,8 @@ void mgpu_arena_free(SegArena* a);
 sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterWords,
                                 hipStream_t stream);
 // Upper bound on the plan arrays of a build over `numSegments` segments
-// holding `keys` keys in total, and on its queue counters.
-size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys);
+// holding `keys` keys in total, in tiles of at least `minTile` keys.
+size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys, uint32_t minTile);
 constexpr size_t kSegCounterWords = 4 * kSegVariants;
 // The lane's side stream and fork/join events, on which the persistent
 // kernels of a plan run beside its device-wide passes.

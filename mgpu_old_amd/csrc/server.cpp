@@ -370,7 +370,7 @@ sortStatus_t SORTAPI sortServerCreate(sortEngine_t engine, unsigned long long hb
         // with it, 62-64 without, at 4 or 8 hardware queues alike).
         slot.lane.noSide = true;
         slot.arenas.resize(kReservedClasses);
-        const size_t payload = mgpu_seg_payload_bound(kMaxBatchRequests, batchKeys);
+        const size_t payload = mgpu_seg_payload_bound(kMaxBatchRequests, batchKeys, (uint32_t)tile);
         for (SegArena& a : slot.arenas)
             if (!st) st = mgpu_arena_reserve(&a, payload, kSegCounterWords, slot.lane.stream);
         if (!st && hipStreamSynchronize(slot.lane.stream) != hipSuccess) st = SORT_STATUS_DEVICE_ERROR;
