@@ -379,3 +379,42 @@ def test_repeat_determinism(engine):
             first = out
         else:
             assert np.array_equal(out[0], first[0]) and np.array_equal(out[1], first[1])
+
+
+@pytest.mark.parametrize("offset", [1, 2, 3])
+@pytest.mark.parametrize("n", [100_003, (1 << 22) + 5, (1 << 24) + 3])
+def test_sub_array_pointers(engine, oracle, offset, n):
+    """Keys and values that start 4, 8 or 12 bytes into larger buffers (not
+    16-byte aligned: the count reads them without vector loads), below and
+    above the size where the count switches kernels and where passes split
+    their tiles into one lookback chain per XCD; nothing outside
+    [offset, offset + n) of either buffer changes."""
+    import torch
+    keys = workloads.uniform_keys(n, 700 + offset)
+    vals = workloads.uniform_keys(n, 800 + offset)
+    rk, (rv,) = oracle.stable_sort(keys, [vals])
+    pad = 8
+    bufs = []
+    for a in (keys, vals):
+        t = torch.full((n + 2 * pad,), 0x5A5A5A5A, dtype=torch.int32, device="cuda")
+        t[offset:offset + n].copy_(torch.from_numpy(a.view(np.int32)))
+        bufs.append(t)
+    alt = [torch.full((n + 2 * pad,), 0x3C3C3C3C, dtype=torch.int32, device="cuda") for _ in range(2)]
+    d = mg.SortData()
+    d.maxElements = d.numElements = n
+    d.valueCount, d.firstBit, d.endBit = 1, 0, 32
+    d.keys[0], d.keys[1] = bufs[0].data_ptr() + 4 * offset, alt[0].data_ptr() + 4 * offset
+    d.values[0][0], d.values[0][1] = bufs[1].data_ptr() + 4 * offset, alt[1].data_ptr() + 4 * offset
+    torch.cuda.synchronize()
+    engine.sort_array(d)
+    torch.cuda.synchronize()
+    # Four passes: the result is back in the caller's first buffers.
+    assert d.parity == 0 and d.keys[0] == bufs[0].data_ptr() + 4 * offset
+    out_k = bufs[0].cpu().numpy().view(np.uint32)
+    out_v = bufs[1].cpu().numpy().view(np.uint32)
+    assert np.array_equal(out_k[offset:offset + n], rk)
+    assert np.array_equal(out_v[offset:offset + n], rv)
+    for t, fill in ((bufs[0], 0x5A5A5A5A), (bufs[1], 0x5A5A5A5A), (alt[0], 0x3C3C3C3C),
+                    (alt[1], 0x3C3C3C3C)):
+        h = t.cpu().numpy().view(np.uint32)
+        assert np.all(h[:offset] == fill) and np.all(h[offset + n:] == fill)
