@@ -19,6 +19,9 @@ Also on the same line:
                 in one graph-replayed launch vs a loop of sortArray calls
   served        Poisson stream through the sort server: p50/p99 latency and
                 Gkeys/s at offered loads of 50% and 80% of measured capacity
+  early_exit    2^28 key + value with earlyExit on and off: random input (the
+                cost of the check), sorted, nearly sorted (0.5% of the 1K-key
+                blocks of sorted keys shuffled) and in top-byte order
 """
 import argparse
 import json
@@ -48,6 +51,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-segmented", action="store_true")
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-early-exit", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log2n", type=int, default=25)
     ap.add_argument("--served-requests", type=int, default=600)
@@ -134,15 +138,19 @@ def verify_on_device(torch, keys_in, keys_out, perm, first_bit, end_bit):
     return ok
 
 
-def time_workload(torch, eng, wl, steps, warmup, timing=False):
+def time_workload(torch, eng, wl, steps, warmup, timing=False, early_exit=False):
     """Wall time per sort of `steps` sorts (synchronized on both sides), and
     optionally the per-kernel event timings of the same sorts."""
     for i in range(warmup):
-        eng.sort_array(wl.data(i))
+        d = wl.data(i)
+        d.earlyExit = early_exit
+        eng.sort_array(d)
     torch.cuda.synchronize()
     eng.check_device()
     eng.enable_timing(timing)
     datas = [wl.data(warmup + i) for i in range(steps)]
+    for d in datas:
+        d.earlyExit = early_exit
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for d in datas:
@@ -187,6 +195,65 @@ def run_workload(torch, eng, name, keys, values, value_count, first_bit, end_bit
     torch.cuda.empty_cache()
     log(f"[bench] {name}: {out['gkeys_per_s']:.2f} Gkeys/s, {out['ms_per_sort']:.3f} ms, "
         f"sort roofline {out['sort_roofline']['frac']:.3f}")
+    return out
+
+
+def early_exit_bench(torch, eng, log2n, steps):
+    """earlyExit on 2^log2n key + value sorts: each input timed with the
+    flag off and on (same copies, same verification)."""
+    import mgpu_old_amd as mg
+    from mgpu_old_amd import workloads
+    n = 1 << log2n
+    keys = workloads.uniform_keys_torch(n, 2029, "cuda", torch=torch)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+
+    def sorted_by(k, first_bit, end_bit):
+        # Input generation with the sort itself (checked by the runs below).
+        wl = Workload(torch, k, [], 0, first_bit, end_bit, 1)
+        d = wl.data(0)
+        eng.sort_array(d)
+        torch.cuda.synchronize()
+        out = wl.result(d)[0].clone()
+        del wl
+        return out
+
+    srt = sorted_by(keys, 0, 32)
+    nearly = srt.clone()
+    blocks = nearly[: n // 1024 * 1024].view(-1, 1024)
+    u = workloads.words_torch(2030, 0, blocks.shape[0], "cuda", torch).long() & 0xFFFFFFFF
+    pick = torch.nonzero(u < int(0.005 * 4294967296.0)).flatten()
+    rnd = workloads.words_torch(2031, 0, pick.numel() * 1024, "cuda", torch).view(-1, 1024)
+    order = torch.argsort(rnd, dim=1)
+    blocks[pick] = torch.gather(blocks[pick], 1, order)
+    top = sorted_by(keys, 24, 32)
+    cases = [("random", keys, 0, 32), ("sorted", srt, 0, 32),
+             ("nearly sorted (0.5% of 1K blocks shuffled)", nearly, 0, 32),
+             ("nearly sorted, bits [18,26)", nearly, 18, 26),
+             ("top-byte order", top, 0, 32)]
+    out = {"n": n, "value_count": 1, "shuffled_blocks": int(pick.numel()), "cases": []}
+    for name, k, fb, eb in cases:
+        row = {"input": name, "bits": [fb, eb]}
+        for flag in (False, True):
+            wl = Workload(torch, k, [vals], 1, fb, eb, steps + 2)
+            sec, datas, kt = time_workload(torch, eng, wl, steps, 2, timing=False,
+                                           early_exit=flag)
+            k_out, v_out = wl.result(datas[0])
+            ok = verify_on_device(torch, wl.src_k, k_out, v_out, fb, eb)
+            row["on" if flag else "off"] = {"ms_per_sort": sec * 1e3,
+                                            "gkeys_per_s": n / sec / 1e9,
+                                            "parity": datas[0].parity,
+                                            "bit_exact_properties": ok}
+            del wl
+            torch.cuda.empty_cache()
+        row["speedup_on_vs_off"] = row["off"]["ms_per_sort"] / row["on"]["ms_per_sort"]
+        out["cases"].append(row)
+        log(f"[bench] earlyExit {name} [{fb},{eb}): off {row['off']['ms_per_sort']:.3f} ms, "
+            f"on {row['on']['ms_per_sort']:.3f} ms (parity {row['on']['parity']}), "
+            f"verify {row['off']['bit_exact_properties'] and row['on']['bit_exact_properties']}")
+    rnd_row = out["cases"][0]
+    out["cost_on_random"] = rnd_row["on"]["ms_per_sort"] / rnd_row["off"]["ms_per_sort"] - 1
+    del keys, vals, srt, nearly, top
+    torch.cuda.empty_cache()
     return out
 
 
@@ -452,6 +519,7 @@ def main():
     for e in extra:
         e.pop("kernels", None)
 
+    ee = None if args.no_early_exit else early_exit_bench(torch, eng, args.log2n, 5)
     seg = None if args.no_segmented else segmented_bench(torch, eng, 10)
     served = None if args.no_served else served_bench(torch, eng, args.served_requests)
 
@@ -483,6 +551,7 @@ def main():
         "cpu_baseline": cpu,
         "cpu_baselines": cpu_runs,
         "extra": extra,
+        "early_exit": ee,
         "segmented": seg,
         "served": served,
     }

@@ -165,7 +165,17 @@ sortStatus_t SORTAPI sortFreeData(sortData_d* data);
 /* Replaces inc/mgpusort.h:167.  Stable sort of keys[0][0, numElements) by the
  * bit field [firstBit, endBit), values following the same permutation.  The
  * work is queued on the engine's stream (the null stream unless
- * sortSetStream chose another); the call returns once it is queued. */
+ * sortSetStream chose another); the call returns once it is queued.
+ *
+ * Kernels check themselves (a lookback wait that gives up, a scatter offset
+ * out of range, a tile or segment left unfinished) and record a failure in an
+ * error word of the engine.  A failure of earlier, asynchronous work that has
+ * not been reported yet makes sortArray, sortArrayEx and
+ * sortSegmentedPlanExecute return SORT_STATUS_KERNEL_ERROR before they queue
+ * anything (data unchanged; the word is cleared, so a second call runs).
+ * sortHost, sortDevice, sortSegmentedArray, sortSegmentedBatch and
+ * sortCheckDevice wait for the stream and return it for their own work too;
+ * the sort server reports it per batch in sortCompletion_d.status. */
 sortStatus_t SORTAPI sortArray(sortEngine_t engine, sortData_t data);
 
 /* Replaces inc/mgpusort.h:169-170.  Same as sortArray with the pass split
@@ -185,7 +195,9 @@ sortStatus_t SORTAPI sortDevice(sortEngine_t engine, CUdeviceptr keys, CUdevicep
 
 /* ---- Additions with no counterpart in the reference. ---------------------- */
 
-/* Stream the engine queues its work on (a hipStream_t; NULL = null stream). */
+/* Stream the engine queues its work on (a hipStream_t; NULL = null stream).
+ * Work queued after the switch waits for what was queued on the old stream
+ * (an event), since both use the engine's scratch. */
 sortStatus_t SORTAPI sortSetStream(sortEngine_t engine, void* stream);
 sortStatus_t SORTAPI sortGetStream(sortEngine_t engine, void** stream);
 
@@ -198,10 +210,11 @@ int SORTAPI sortPlanPasses(int numBits, int* widths);
  * up exactly as sortArray would leave it if called on that segment alone with
  * the same bit range and value count: stable by the bit field, values
  * following, the result on side (sortPlanPasses(endBit - firstBit) & 1) of its
- * buffers.  Segments of up to 2^18 elements are sorted by one wave or one
- * workgroup each (digit width and tile shape per size bucket from the tuning
- * table, kernelPath + "mgpusort_seg_tuning.txt"); larger ones run the
- * device-wide passes. */
+ * buffers.  Segments of up to 4096 elements are sorted by one wave or one
+ * workgroup each (digit width and keys per thread per size bucket from the
+ * tuning table, kernelPath + "mgpusort_seg_tuning.txt"); larger ones run the
+ * device-wide passes, all of them in one launch per pass.  With
+ * SORT_VALUE_INDEX both sides of values[0] must be given. */
 
 /* Sorts the segments [segmentStarts[i], segmentStarts[i+1]) of data's
  * buffers (host array of numSegments + 1 nondecreasing offsets, the last at
@@ -227,7 +240,10 @@ sortStatus_t SORTAPI sortSegmentedBatch(sortEngine_t engine, const sortSegment_d
 /* Reusable form of sortSegmentedArray: the work lists are built once and the
  * launches captured in a hipGraph; each execute replays it on the engine's
  * stream (asynchronously) and swaps data's buffers as sortArray does.  data
- * must hold the buffers given at creation, in either order. */
+ * must hold the buffers given at creation, in either order.  The plan owns
+ * the scratch of its device-wide passes (allocated at creation, freed by
+ * destroy), so later sorts on the engine, which grow and replace the
+ * engine's own scratch, cannot disturb a replay. */
 struct sortSegPlan_d;
 typedef struct sortSegPlan_d* sortSegPlan_t;
 sortStatus_t SORTAPI sortSegmentedPlanCreate(sortEngine_t engine, const sortData_d* data,
@@ -290,7 +306,7 @@ sortStatus_t SORTAPI sortServerDestroy(sortServer_t server);
 /* Waits for the engine's stream and reports SORT_STATUS_KERNEL_ERROR if a
  * kernel detected an inconsistency (a scatter offset out of range, a
  * lookback wait that gave up, or a segment left without a workgroup on its
- * XCD) since the last call; SUCCESS otherwise. */
+ * XCD) that no call has reported yet; SUCCESS otherwise. */
 sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine);
 
 /* What the engine chose at creation, and the last device inconsistency code

@@ -29,6 +29,8 @@ CODE_OBJECTS = {
     "mgpusort_seg.hsaco": ["segsort.hip"],
 }
 HOST_SOURCES = ["engine.cpp", "loader.cpp", "segmented.cpp", "server.cpp"]
+# Host code with device code of its own, linked into libmgpusort.so.
+HIP_HOST_SOURCES = []
 
 
 def _run(cmd):
@@ -44,11 +46,18 @@ def build(verbose=True):
               "--no-gpu-bundle-output", *DEVICE_FLAGS,
               "-I", CSRC, *[os.path.join(CSRC, s) for s in srcs],
               "-o", os.path.join(LIB, obj)])
+    objects = []
+    for src in HIP_HOST_SOURCES:
+        obj = os.path.join(LIB, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-c",
+              "-fvisibility=hidden", *DEVICE_FLAGS,
+              "-I", CSRC, "-I", INCLUDE, os.path.join(CSRC, src), "-o", obj])
+        objects.append(obj)
     # Host code only: plain g++ against the HIP runtime, no device pass.
     _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
           "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__",
           "-I", CSRC, "-I", INCLUDE, "-I", os.path.join(ROCM, "include"),
-          *[os.path.join(CSRC, s) for s in HOST_SOURCES],
+          *[os.path.join(CSRC, s) for s in HOST_SOURCES], *objects,
           "-o", os.path.join(LIB, "libmgpusort.so"),
           "-L", os.path.join(ROCM, "lib"), f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}",
           "-lamdhip64", "-ldl", "-pthread"])

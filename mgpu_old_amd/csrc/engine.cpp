@@ -170,6 +170,19 @@ sortStatus_t mgpu_ensure_multi(SortLane& L, uint32_t arrays) {
 }
 
 sortStatus_t mgpu_lane_init(SortLane* l) {
+    // The error word: coherent pinned memory, so a kernel's store reaches the
+    // host without a copy and nothing on the normal path pays for it.
+    void* word = nullptr;
+    if (hipHostMalloc(&word, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        l->errorWord = nullptr;
+        return SORT_STATUS_HOST_ALLOC_FAILED;
+    }
+    l->errorWord = static_cast<uint32_t*>(word);
+    std::memset(word, 0, 64);
+    void* dev = nullptr;
+    if (hipHostGetDevicePointer(&dev, word, 0) != hipSuccess || !dev)
+        return SORT_STATUS_DEVICE_ERROR;
+    l->errorDev = static_cast<uint32_t*>(dev);
     const size_t words = 2 * kLaneHistWords + MGPU_LANE_COUNTERS + kLaneSegCountWords;
     if (hipMalloc(&l->fixed, words * sizeof(uint32_t)) != hipSuccess) {
         l->fixed = nullptr;
@@ -195,6 +208,8 @@ void mgpu_lane_free(SortLane* l) {
     if (l->side) (void)hipStreamDestroy(l->side);
     l->fork = l->join = nullptr;
     l->side = nullptr;
+    if (l->streamSwitch) (void)hipEventDestroy(l->streamSwitch);
+    l->streamSwitch = nullptr;
     if (l->multi) (void)hipFree(l->multi);
     l->multi = nullptr;
     l->multiArrays = 0;
@@ -203,6 +218,19 @@ void mgpu_lane_free(SortLane* l) {
     l->lookback = nullptr;
     l->lookbackWords = 0;
     l->fixed = nullptr;
+    if (l->errorWord) (void)hipHostFree(l->errorWord);
+    l->errorWord = l->errorDev = nullptr;
+}
+
+uint32_t mgpu_lane_peek_error(const SortLane& l) {
+    return l.errorWord ? __atomic_load_n(l.errorWord, __ATOMIC_ACQUIRE) : 0u;
+}
+
+sortStatus_t mgpu_lane_take_error(SortLane& l) {
+    if (!l.errorWord || __atomic_load_n(l.errorWord, __ATOMIC_ACQUIRE) == 0)
+        return SORT_STATUS_SUCCESS;
+    return __atomic_exchange_n(l.errorWord, 0u, __ATOMIC_ACQ_REL) ? SORT_STATUS_KERNEL_ERROR
+                                                                  : SORT_STATUS_SUCCESS;
 }
 
 // ---------------------------------------------------------------------------
@@ -236,10 +264,11 @@ hipEvent_t pooled_event(sortEngine_d* e) {
 }  // namespace
 
 sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, unsigned grid,
-                         unsigned block, void* args) {
+                         unsigned block, void* args, bool timed) {
     void* params[] = {args};
     hipEvent_t start = nullptr, stop = nullptr;
-    if (e->timing) {
+    timed = timed && e->timing;
+    if (timed) {
         start = pooled_event(e);
         stop = pooled_event(e);
         if (!start || !stop || hipEventRecord(start, stream) != hipSuccess)
@@ -247,7 +276,7 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
     }
     hipError_t r = hipModuleLaunchKernel(f, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr);
     if (r != hipSuccess) return SORT_STATUS_LAUNCH_ERROR;
-    if (e->timing) {
+    if (timed) {
         if (hipEventRecord(stop, stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
         e->timed.push_back({f, start, stop});
     }
@@ -271,8 +300,17 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     const uint32_t tile = var.tile();
     const uint32_t numTiles = (n + tile - 1) / tile;
     const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
-    sortStatus_t st = ensure_lookback(L, lbWords);
-    if (st) return st;
+    sortStatus_t st = SORT_STATUS_SUCCESS;
+    const bool timed = !L.noAlloc;
+    if (L.noAlloc) {
+        // Scratch of a fixed size, filled by nothing that allocates or waits.
+        if (lbWords > L.lookbackWords) return SORT_STATUS_CONFIG_NOT_SUPPORTED;
+        if (L.histDirty) return SORT_STATUS_INTERNAL_ERROR;
+        earlyExit = false;
+    } else {
+        st = ensure_lookback(L, lbWords);
+        if (st) return st;
+    }
     // One segment per XCD (each with a lookback chain in its XCD's L2) for
     // large sorts; else one chain across the device.  digit_count counts the
     // segments of the input order, which only the first executed pass reads;
@@ -345,16 +383,26 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         grid = ca.chunks_per_seg * segs;
     }
     L.histDirty = true;
-    st = mgpu_launch(e, L.stream, cols ? e->fnCountCols : e->fnCount, grid, countThreads, &ca);
+    st = mgpu_launch(e, L.stream, cols ? e->fnCountCols : e->fnCount, grid, countThreads, &ca,
+                     timed);
     if (st) return st;
 
-    // Early exit (the role of the reference's sortDetectCounters readback,
-    // mgpusort.cpp:433-451, for all passes at once): a pass whose digit is the
-    // same for every key is the identity and is skipped; already-sorted input
-    // skips every pass.
+    // Early exit (the role of the reference's sortDetectCounters readback and
+    // its codes 1-3, mgpusort.cpp:433-451, for all passes at once):
+    //   * already-sorted input (by the whole bit field) skips every pass;
+    //   * a pass whose digit is the same for every key is the identity;
+    //   * so is the first pass when the input is already in its digit's
+    //     order (digit_count counts that digit's descents, as the reference's
+    //     count checks the current digit);
+    //   * pass 0, when it runs, takes the early-exit kernel (waves already in
+    //     digit order rank by position, without LDS atomics) when its digit
+    //     has at most 3 descents per 1024-key wave: were they spread evenly,
+    //     5% of the waves would be in order, and clustered disorder (nearly
+    //     sorted input) leaves more.
     bool run[MGPU_MAX_PASSES];
     for (int p = 0; p < numPasses; ++p) run[p] = true;
     bool sorted = false;
+    int eePass = -1;
     if (earlyExit) {
         const size_t hw = (size_t)segs * MGPU_MAX_PASSES * MGPU_RADIX;
         std::vector<uint32_t> h(hw + MGPU_LANE_COUNTERS);
@@ -365,8 +413,9 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
             hipStreamSynchronize(L.stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
         sorted = h[hw + MGPU_UNSORTED_SLOT] == 0;
+        bool leading = true;  // no pass has run yet: the input is as given
+        const uint32_t firstDescents = h[hw + MGPU_UNSORTED_SLOT + 1];
         for (int p = 0; p < numPasses; ++p) {
-            // The pass is the identity when one digit holds every key.
             uint32_t most = 0;
             for (uint32_t d = 0; d < MGPU_RADIX; ++d) {
                 uint32_t c = 0;
@@ -374,7 +423,16 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
                     c += h[(x * MGPU_MAX_PASSES + p) * MGPU_RADIX + d];
                 most = std::max(most, c);
             }
-            run[p] = !sorted && most != n;
+            // Descents are counted for pass 0's digit only: what a later
+            // leading pass would find in the input is not known.
+            const bool inOrder = p == 0 && firstDescents == 0;
+            run[p] = !sorted && most != n && !(leading && inOrder);
+            if (run[p] && leading) {
+                leading = false;
+                const uint64_t waves = (n + 1023) / 1024;
+                const bool few = p == 0 && firstDescents <= 3 * waves;
+                if (e->eePolicy == 1 || (e->eePolicy < 0 && few)) eePass = p;
+            }
         }
     }
 
@@ -388,7 +446,7 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     sa.seg_count = L.segCount;
     sa.seg_count_words = segs > 1 ? (uint32_t)kLaneSegCountWords : 0u;
     sa.pad_ = 0;
-    st = mgpu_launch(e, L.stream, e->fnScan, (unsigned)numPasses, 256, &sa);
+    st = mgpu_launch(e, L.stream, e->fnScan, (unsigned)numPasses, 256, &sa, timed);
     if (st) return st;
     L.histDirty = false;
 
@@ -417,7 +475,8 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         pa.lookback = L.lookback + (uint64_t)p * numTiles * MGPU_RADIX;
         pa.tile_counter = L.tileCounters + MGPU_TILE_COUNTER(p, 0);
         pa.finished = L.tileCounters + MGPU_FINISHED_SLOT + p;
-        pa.error_flag = e->errorFlag;
+        pa.error_flag = L.errorDev;
+        pa.spin_limit = e->spinLimit;
         pa.n = n;
         pa.num_tiles = numTiles;
         pa.shift = (uint32_t)shiftBits;
@@ -439,9 +498,11 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
             // Segment workgroups loop over their XCD's tiles: as many as fit
             // at once (at least one per XCD).
             const uint32_t g = std::max(segs, std::min(numTiles, var.xsegGrid));
-            st = mgpu_launch(e, L.stream, var.xseg[mode], g, var.blockThreads, &pa);
+            hipFunction_t f = p == eePass && var.xsegEE[mode] ? var.xsegEE[mode] : var.xseg[mode];
+            st = mgpu_launch(e, L.stream, f, g, var.blockThreads, &pa, timed);
         } else {
-            st = mgpu_launch(e, L.stream, var.pass[mode], numTiles, var.blockThreads, &pa);
+            hipFunction_t f = p == eePass && var.passEE[mode] ? var.passEE[mode] : var.pass[mode];
+            st = mgpu_launch(e, L.stream, f, numTiles, var.blockThreads, &pa, timed);
         }
         if (st) return st;
         ++done;
@@ -454,17 +515,15 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         ia.n = n;
         ia.pad_ = 0;
         uint32_t g = std::min<uint32_t>((n + 255) / 256, (uint32_t)e->numCUs * 8);
-        st = mgpu_launch(e, L.stream, e->fnIota, g, 256, &ia);
+        st = mgpu_launch(e, L.stream, e->fnIota, g, 256, &ia, timed);
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;
 }
 
-namespace {
-
 // Validation shared by sortArray and sortArrayEx (mgpusort.cpp:360-364,
 // 609-611, 625).
-sortStatus_t validate(sortEngine_t engine, sortData_t data) {
+sortStatus_t mgpu_validate_data(sortEngine_d* engine, const sortData_d* data) {
     if (!engine || !data) return SORT_STATUS_INVALID_VALUE;
     if (data->numElements < 0 || data->numElements > data->maxElements)
         return SORT_STATUS_INVALID_VALUE;
@@ -478,7 +537,8 @@ sortStatus_t validate(sortEngine_t engine, sortData_t data) {
 // sortArrayFromList (mgpusort.cpp:560-601) without the end-key handling:
 // runs the passes, swaps every buffer pair once per executed pass and toggles
 // parity, exactly as sortPass does (mgpusort.cpp:518-526).
-sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, const int* widths) {
+sortStatus_t mgpu_sort_data(sortEngine_d* e, SortLane& lane, sortData_d* data, int numPasses,
+                            const int* widths) {
     const uint32_t n = (uint32_t)data->numElements;
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     const int carried = data->valueCount == -1 ? 1 : data->valueCount;
@@ -497,7 +557,7 @@ sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, co
         for (int s = 0; s < 2; ++s) valBuf[v][s] = reinterpret_cast<uint32_t*>(data->values[v][s]);
 
     int executed = 0;
-    sortStatus_t st = mgpu_sort_passes(e, e->lane, n, data->firstBit, numPasses, widths,
+    sortStatus_t st = mgpu_sort_passes(e, lane, n, data->firstBit, numPasses, widths,
                                        data->valueCount, keyBuf, valBuf, data->earlyExit,
                                        &executed);
     if (st) return st;
@@ -507,6 +567,12 @@ sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, co
     }
     data->parity ^= (executed & 1);
     return SORT_STATUS_SUCCESS;
+}
+
+namespace {
+
+sortStatus_t sort_with_widths(sortEngine_t e, sortData_t data, int numPasses, const int* widths) {
+    return mgpu_sort_data(e, e->lane, data, numPasses, widths);
 }
 
 }  // namespace
@@ -582,21 +648,29 @@ sortStatus_t SORTAPI sortDestroyEngine(sortEngine_t engine) { return sortRelease
 
 sortStatus_t SORTAPI sortSetStream(sortEngine_t engine, void* stream) {
     if (!engine) return SORT_STATUS_INVALID_VALUE;
-    engine->lane.stream = reinterpret_cast<hipStream_t>(stream);
+    hipStream_t next = reinterpret_cast<hipStream_t>(stream);
+    SortLane& L = engine->lane;
+    if (next != L.stream) {
+        // Work still queued on the old stream uses the lane's scratch (digit
+        // counts, lookback words, tile counters) that the next sort reuses:
+        // the new stream waits for it.
+        if (!L.streamSwitch &&
+            hipEventCreateWithFlags(&L.streamSwitch, hipEventDisableTiming) != hipSuccess) {
+            L.streamSwitch = nullptr;
+            return SORT_STATUS_DEVICE_ERROR;
+        }
+        if (hipEventRecord(L.streamSwitch, L.stream) != hipSuccess ||
+            hipStreamWaitEvent(next, L.streamSwitch, 0) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
+        L.stream = next;
+    }
     return SORT_STATUS_SUCCESS;
 }
 
 sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine) {
     if (!engine) return SORT_STATUS_INVALID_VALUE;
     if (hipStreamSynchronize(engine->lane.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
-    uint32_t flag = 0;
-    if (hipMemcpy(&flag, engine->errorFlag, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
-    if (!flag) return SORT_STATUS_SUCCESS;
-    if (hipMemsetAsync(engine->errorFlag, 0, sizeof(uint32_t), engine->lane.stream) != hipSuccess ||
-        hipStreamSynchronize(engine->lane.stream) != hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
-    return SORT_STATUS_KERNEL_ERROR;
+    return mgpu_lane_take_error(engine->lane);
 }
 
 sortStatus_t SORTAPI sortTransformKeys(sortEngine_t engine, sortDevicePtr_t keys, int numElements,
@@ -626,10 +700,7 @@ sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info)
     info->smallTileKeys = (int)engine->small.tile();
     info->largeTileKeys = (int)engine->large.tile();
     if (hipStreamSynchronize(engine->lane.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
-    uint32_t flag = 0;
-    if (hipMemcpy(&flag, engine->errorFlag, sizeof(flag), hipMemcpyDeviceToHost) != hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
-    info->deviceError = flag;
+    info->deviceError = mgpu_lane_peek_error(engine->lane);
     return SORT_STATUS_SUCCESS;
 }
 
@@ -675,12 +746,16 @@ sortStatus_t SORTAPI sortGetStream(sortEngine_t engine, void** stream) {
     return SORT_STATUS_SUCCESS;
 }
 
-// sortArray (mgpusort.cpp:607-616).
+// sortArray (mgpusort.cpp:607-616).  A self-check failure of earlier work on
+// the engine's stream that has not been reported yet is returned here, before
+// anything is queued (the sort itself is asynchronous, so its own failures
+// surface at the next call, at sortCheckDevice, or in a synchronous wrapper).
 sortStatus_t SORTAPI sortArray(sortEngine_t engine, sortData_t data) {
     if (!engine || !data) return SORT_STATUS_INVALID_VALUE;
+    if (mgpu_lane_take_error(engine->lane)) return SORT_STATUS_KERNEL_ERROR;
     int numBits = data->endBit - data->firstBit;
     if (!numBits) return SORT_STATUS_SUCCESS;  // as mgpusort.cpp:610
-    sortStatus_t st = validate(engine, data);
+    sortStatus_t st = mgpu_validate_data(engine, data);
     if (st) return st;
     int widths[MGPU_MAX_PASSES];
     int passes = mgpu_plan_passes(numBits, MGPU_RADIX_BITS, widths);
@@ -696,10 +771,11 @@ sortStatus_t SORTAPI sortArrayEx(sortEngine_t engine, sortData_t data, int numSo
     if ((numSortThreads != 128 && numSortThreads != 256) || valuesPerThread != 8)
         return SORT_STATUS_INVALID_VALUE;
     if (data->numElements > data->maxElements) return SORT_STATUS_INVALID_VALUE;
+    if (mgpu_lane_take_error(engine->lane)) return SORT_STATUS_KERNEL_ERROR;
     if (bitPass <= 0 || bitPass > 6) bitPass = 6;
     int numBits = data->endBit - data->firstBit;
     if (!numBits) return SORT_STATUS_SUCCESS;
-    sortStatus_t st = validate(engine, data);
+    sortStatus_t st = mgpu_validate_data(engine, data);
     if (st) return st;
     if (bitPass > numBits) bitPass = numBits;
     int widths[32];
@@ -826,6 +902,7 @@ sortStatus_t SORTAPI sortHost(sortEngine_t engine, unsigned int* keys, unsigned 
             st = SORT_STATUS_DEVICE_ERROR; break;
         }
         if (hipStreamSynchronize(s) != hipSuccess) st = SORT_STATUS_DEVICE_ERROR;
+        else st = mgpu_lane_take_error(engine->lane);
     } while (0);
     (void)hipStreamSynchronize(s);
     sortFreeData(&d);
@@ -862,6 +939,7 @@ sortStatus_t SORTAPI sortDevice(sortEngine_t engine, CUdeviceptr keys, CUdevicep
             st = SORT_STATUS_DEVICE_ERROR;
     }
     if (hipStreamSynchronize(engine->lane.stream) != hipSuccess && !st) st = SORT_STATUS_DEVICE_ERROR;
+    if (!st) st = mgpu_lane_take_error(engine->lane);
     // Free only what sortAllocData added: the caller's buffers sit at
     // [parity] after the sort.
     int p = d.parity & 1;

@@ -20,6 +20,12 @@ struct PassVariant {
     hipFunction_t pass[4] = {};   // mgpu_rank_scatter_b*_i*_r*_v<mode>: one device-wide chain
     hipFunction_t xseg[4] = {};   // mgpu_rank_scatter_b*_i*_r*_x_v<mode>: a chain per XCD segment
     hipFunction_t multi[4] = {};  // mgpu_multi_scatter_b*_i*_r*_v<mode>
+    hipFunction_t multiX[4] = {}; // mgpu_multi_scatter_b*_i*_r*_x_v<mode>: chains per XCD
+    uint32_t multiXGrid = 0;      // workgroups of a multiX launch: all that fit
+    // Early-exit forms (atomic-add ranking only; null when not built for the
+    // shape): mgpu_rank_scatter_b*_i*_r1_ee_v<mode> and ..._r1_x_ee_v<mode>.
+    hipFunction_t passEE[4] = {};
+    hipFunction_t xsegEE[4] = {};
     uint32_t xsegGrid = 0;        // workgroups of an xseg launch: all that fit on the device
     uint32_t tile() const { return blockThreads * itemsPerThread; }
 };
@@ -55,10 +61,29 @@ struct SortLane {
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     bool noSide = false;  // never fork: everything on `stream` (sort server slots)
+    // Error word of the lane's kernels: pinned host memory mapped into the
+    // device, which a kernel sets (system-scope store) when a self-check
+    // fails: 1 a lookback wait gave up, 2 a tile counter left dirty, 4 a
+    // scatter offset out of range, 5 a segment left without a workgroup on
+    // its XCD.  The host reads it without a copy.
+    uint32_t* errorWord = nullptr;  // host address
+    uint32_t* errorDev = nullptr;   // the same word as the kernels address it
+    hipEvent_t streamSwitch = nullptr;  // orders a new stream after the old (sortSetStream)
+    // A view over caller-provided scratch (sortArrayNoAlloc): the passes
+    // never allocate, clear with a memset, read back or time anything; a sort
+    // whose lookback words do not fit lookbackWords is refused.
+    bool noAlloc = false;
 };
 
 sortStatus_t mgpu_lane_init(SortLane* l);
 void mgpu_lane_free(SortLane* l);
+// The lane's error code (0: none) without clearing it.
+uint32_t mgpu_lane_peek_error(const SortLane& l);
+// SORT_STATUS_KERNEL_ERROR, and the word cleared, when a kernel of the lane
+// has set it since the last take; SORT_STATUS_SUCCESS otherwise.  Only work
+// that has finished is seen: callers synchronize first when they need all of
+// it.
+sortStatus_t mgpu_lane_take_error(SortLane& l);
 
 struct sortEngine_d {
     int refcount = 1;  // sortIncEngine / sortReleaseEngine (not atomic, as util/util.h:84-98)
@@ -73,7 +98,8 @@ struct sortEngine_d {
     hipFunction_t fnScan = nullptr;
     hipFunction_t fnIota = nullptr;
     hipFunction_t fnKeyTransform = nullptr;
-    hipFunction_t fnMultiCount = nullptr;
+    hipFunction_t fnMultiCount = nullptr;      // 256 threads, per-wave histograms
+    hipFunction_t fnMultiCountCols = nullptr;  // 1024 threads, lane-column histograms
     hipFunction_t fnMultiScan = nullptr;
     // Tile shapes: `large` once a pass has a large tile for every
     // kCUsPerLargeTile CUs, `small` below that (more, smaller tiles keep the
@@ -87,8 +113,13 @@ struct sortEngine_d {
     uint32_t xcdSegs = 1;
     bool xcdForce = false;
 
-    // Set by kernels that detect an inconsistency (sortCheckDevice).
-    uint32_t* errorFlag = nullptr;
+    // Polls of a lookback word before a pass gives up (MGPUSORT_SPIN_LIMIT
+    // at creation; 0 makes every lookback give up at once, for tests of the
+    // error path).
+    uint32_t spinLimit = 1u << 22;
+    // Early-exit kernel for the first pass that runs (MGPUSORT_EE at
+    // creation): -1 by the descent counts, 0 never, 1 always.
+    int eePolicy = -1;
 
     // sortArray tuning table (kernelPath + "mgpusort_sort_tuning.txt", written
     // by the autotuner): per value class (MGPU_SORT_CLASS_*), entries sorted
@@ -100,12 +131,14 @@ struct sortEngine_d {
     };
     std::vector<SortTuneEntry> sortTable[4];
 
-    // Segmented sort: kernel variants [radix bits - 4][items 16?] and the
-    // size-bucket tuning table.
+    // Segmented sort: kernel variants [seg_variant_index] (null where not
+    // built), workgroups per CU of each, the size-bucket tuning table and the
+    // device-wide passes' settings.
     hipModule_t segModule = nullptr;
-    hipFunction_t segFn[kSegMaxBits - kSegMinBits + 1][2] = {};
-    int segOccupancy[kSegMaxBits - kSegMinBits + 1][2] = {};
+    hipFunction_t segFn[kSegVariants] = {};
+    int segOccupancy[kSegVariants] = {};
     std::vector<SegTuneEntry> segTable;
+    SegWideTune segWide;
 
     // Per-kernel timing (sortEnableTiming / sortReadTiming).
     struct TimedLaunch {
@@ -127,8 +160,9 @@ void mgpu_engine_destroy(sortEngine_d* e);
 // pass count and fills widths (narrower passes first).
 int mgpu_plan_passes(int numBits, int maxWidth, int* widths);
 
+// timed = false: never event-timed, even with sortEnableTiming on.
 sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, unsigned grid,
-                         unsigned block, void* args);
+                         unsigned block, void* args, bool timed = true);
 
 // A pass over `keys` keys in all uses the large tile shape once it has a
 // large tile for every this many CUs (2^21 keys on 256 CUs), else the small
@@ -176,3 +210,11 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& lane, uint32_t n, int f
                               uint32_t* const keyBuf[2],
                               uint32_t* const valBuf[MGPU_MAX_VALUES][2],
                               bool earlyExit = false, int* executed = nullptr);
+
+// Argument checks of sortArray / sortArrayEx (mgpusort.cpp:360-364,609-611).
+sortStatus_t mgpu_validate_data(sortEngine_d* e, const sortData_d* data);
+// sortArrayFromList (mgpusort.cpp:560-601) on a lane: runs the passes over
+// data's buffers, swaps every buffer pair once per executed pass and toggles
+// parity (mgpusort.cpp:518-526).
+sortStatus_t mgpu_sort_data(sortEngine_d* e, SortLane& lane, sortData_d* data, int numPasses,
+                            const int* widths);

@@ -80,7 +80,9 @@ struct mgpuCountArgs {
     uint32_t shift[MGPU_MAX_PASSES];
     uint32_t mask[MGPU_MAX_PASSES];
     // Sortedness check (early exit): counts adjacent pairs whose bit field
-    // (k >> check_shift) & check_mask decreases into *unsorted.
+    // (k >> check_shift) & check_mask decreases into unsorted[0], and those
+    // whose digit of the first pass (shift[0], mask[0]) decreases into
+    // unsorted[1].
     uint32_t* unsorted;      // null: no check
     uint32_t check_shift;
     uint32_t check_mask;
@@ -104,9 +106,12 @@ struct mgpuScanArgs {
 // workgroups of pass p that have finished, the early-exit descent count.
 #define MGPU_TILE_COUNTER(p, x) ((p) * MGPU_MAX_SEGS + (x))
 #define MGPU_FINISHED_SLOT (MGPU_MAX_PASSES * MGPU_MAX_SEGS)
+// Early exit: adjacent pairs of the input whose sort bit field decreases
+// (UNSORTED_SLOT), and whose digit of the first pass decreases
+// (UNSORTED_SLOT + 1).
 #define MGPU_UNSORTED_SLOT (MGPU_FINISHED_SLOT + MGPU_MAX_PASSES)
 // Most frequent digit of each pass (segments per XCD only; digit_scan).
-#define MGPU_MODE_SLOT (MGPU_UNSORTED_SLOT + 4)
+#define MGPU_MODE_SLOT (MGPU_UNSORTED_SLOT + 8)
 #define MGPU_LANE_COUNTERS (MGPU_MODE_SLOT + MGPU_MAX_PASSES)
 
 // iota: out[i] = i for i < n (index output of a sort that needs no pass).
@@ -209,7 +214,7 @@ struct mgpuMultiPassArgs {
     const uint32_t* tile_seg;     // [num_tiles]: the array of each global tile
     const uint32_t* digit_start;  // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX]
     uint32_t* lookback;           // [num_tiles][MGPU_RADIX] for this pass, zero on entry
-    uint32_t* tile_counter;       // zero on entry
+    uint32_t* tile_counter;       // [MGPU_MAX_SEGS] (one per XCD), zero on entry
     uint32_t* error_flag;
     uint32_t num_segs;
     uint32_t num_tiles;
@@ -218,6 +223,14 @@ struct mgpuMultiPassArgs {
     uint32_t shift;
     uint32_t mask;
     uint32_t num_values;
+    uint32_t spin_limit;          // polls of a lookback word before the pass gives up
+    // Chains per XCD (kernels built for it only): the arrays are grouped by
+    // XCD, XCD x holding global tiles [xcd_tile_begin[x], xcd_tile_begin[x+1])
+    // of whole arrays, whose lookback chains its workgroups alone read and
+    // write; finished counts the workgroups done (zero on entry).
+    const uint32_t* xcd_tile_begin;  // [num_xcds + 1]
+    uint32_t* finished;
+    uint32_t num_xcds;
     uint32_t pad_;
 };
 
@@ -233,7 +246,7 @@ struct mgpuPassArgs {
     uint32_t* lookback;           // [num_tiles][MGPU_RADIX], zero on entry
     uint32_t* tile_counter;       // this pass's [MGPU_MAX_SEGS] tile counters, zero on entry
     uint32_t* finished;           // this pass's finished-workgroup count, zero on entry
-    uint32_t* error_flag;         // set to 1 if a lookback wait gave up
+    uint32_t* error_flag;         // host-visible word: set when a self-check fails
     uint32_t n;
     uint32_t num_tiles;
     uint32_t shift;
@@ -252,6 +265,11 @@ struct mgpuPassArgs {
     const uint32_t* next_mode;  // most frequent digit of the next pass (with seg_count_out)
     uint32_t next_shift;
     uint32_t next_mask;
+    // Polls of one lookback word before the pass gives up and sets
+    // *error_flag = 1 (0: every lookback gives up at once, which only tests
+    // of the error path ask for).
+    uint32_t spin_limit;
+    uint32_t pad_;
 };
 
 // Segments per XCD with counting for the next pass: at most this many.

@@ -41,8 +41,28 @@ sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, uin
         std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r%u_x_v%d", threads, items,
                       rank, m);
         if (!st) st = get_fn(e, e->radixModule, name, &v->xseg[m]);
+        std::snprintf(name, sizeof(name), "mgpu_multi_scatter_b%u_i%u_r%u_x_v%d", threads, items,
+                      rank, m);
+        if (!st) st = get_fn(e, e->radixModule, name, &v->multiX[m]);
     }
     if (st) return st;
+    // Early-exit forms, built for the default shapes with atomic-add ranking.
+    for (int m = 0; m < 4 && rank == 1; ++m) {
+        char name[96];
+        hipFunction_t f = nullptr;
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r1_ee_v%d", threads, items, m);
+        if (hipModuleGetFunction(&f, e->radixModule, name) == hipSuccess) {
+            v->passEE[m] = f;
+            e->kernelNames[f] = name;
+        }
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r1_x_ee_v%d", threads, items,
+                      m);
+        if (hipModuleGetFunction(&f, e->radixModule, name) == hipSuccess) {
+            v->xsegEE[m] = f;
+            e->kernelNames[f] = name;
+        }
+    }
+    (void)hipGetLastError();
     // Segment workgroups loop over tiles: launch as many as fit at once.
     int perCU = 0;
     if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, v->xseg[MGPU_VALUES_SINGLE],
@@ -50,6 +70,12 @@ sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, uin
         perCU < 1)
         perCU = 1;
     v->xsegGrid = (uint32_t)perCU * (uint32_t)e->numCUs;
+    perCU = 0;
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, v->multiX[MGPU_VALUES_SINGLE],
+                                                           (int)threads, 0) != hipSuccess ||
+        perCU < 1)
+        perCU = 1;
+    v->multiXGrid = (uint32_t)perCU * (uint32_t)e->numCUs;
     return SORT_STATUS_SUCCESS;
 }
 
@@ -137,6 +163,7 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_key_transform", &e->fnKeyTransform);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count", &e->fnMultiCount);
+    if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_count_cols", &e->fnMultiCountCols);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_multi_scan", &e->fnMultiScan);
     uint32_t smallShape[2] = {512, 16}, largeShape[2] = {1024, 16};
     const char* t = std::getenv("MGPUSORT_TILE");
@@ -162,13 +189,10 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     st = load_variant(e, smallShape[0], smallShape[1], e->rankMethod, &e->small);
     if (!st) st = load_variant(e, largeShape[0], largeShape[1], e->rankMethod, &e->large);
     if (st) return st;
-    if (hipMalloc(&e->errorFlag, 16) != hipSuccess) {
-        e->errorFlag = nullptr;
-        return SORT_STATUS_DEVICE_ALLOC_FAILED;
-    }
-    if (hipMemsetAsync(e->errorFlag, 0, 16, e->lane.stream) != hipSuccess ||
-        hipStreamSynchronize(e->lane.stream) != hipSuccess)
-        return SORT_STATUS_DEVICE_ERROR;
+    const char* spin = std::getenv("MGPUSORT_SPIN_LIMIT");
+    if (spin && *spin) e->spinLimit = (uint32_t)std::strtoul(spin, nullptr, 10);
+    const char* ee = std::getenv("MGPUSORT_EE");
+    if (ee && (ee[0] == '0' || ee[0] == '1') && ee[1] == 0) e->eePolicy = ee[0] - '0';
     return SORT_STATUS_SUCCESS;
 }
 
@@ -181,7 +205,6 @@ void mgpu_engine_destroy(sortEngine_d* e) {
     }
     for (hipEvent_t ev : e->eventPool) (void)hipEventDestroy(ev);
     mgpu_lane_free(&e->lane);
-    if (e->errorFlag) (void)hipFree(e->errorFlag);
     if (e->radixModule) (void)hipModuleUnload(e->radixModule);
     mgpu_seg_unload(e);
     delete e;

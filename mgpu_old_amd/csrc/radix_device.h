@@ -22,6 +22,12 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Records a failed self-check in the launch's error word, which lives in
+// host memory mapped into the device (system scope: the host reads it).
+__device__ __forceinline__ void set_error(uint32_t* flag, uint32_t code) {
+    __hip_atomic_store(flag, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // Inclusive scan over the 64 lanes of a wave.
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x, uint32_t lane) {
 #pragma unroll

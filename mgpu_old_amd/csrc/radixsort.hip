@@ -34,9 +34,6 @@
 namespace {
 
 constexpr uint32_t RADIX = MGPU_RADIX;
-// Polls of one lookback word before a pass gives up (about a second); only a
-// broken tile counter could make a wait that long.
-constexpr uint32_t kSpinLimit = 1u << 22;
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
@@ -49,6 +46,24 @@ __device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
 
 __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Records a failed self-check in the launch's error word, which lives in
+// host memory mapped into the device (system scope: the host reads it).
+__device__ __forceinline__ void set_error(uint32_t* flag, uint32_t code) {
+    __hip_atomic_store(flag, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Lane i gets v of lane i - 1, lane 0 gets lane0 (DPP wave_shr:1).
+__device__ __forceinline__ uint32_t prev_lane(uint32_t v, uint32_t lane0) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)lane0, (int)v, 0x138, 0xF, 0xF, false);
+}
+
+// Orders this wave's LDS accesses before and after the call.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Inclusive scan over the 64 lanes of a wave.
@@ -159,8 +174,10 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
     static_assert(COLS == 0 || (COLS == 32 && BLOCK == MGPU_MAX_PASSES * RADIX),
                   "the column layout merges one (pass, digit) per thread");
     __shared__ uint32_t s_hist[WORDS];
+    __shared__ uint32_t s_desc[2];  // early-exit descents of this workgroup
     const uint32_t tid = threadIdx.x, wave = tid >> 6;
     for (uint32_t i = tid; i < WORDS; i += BLOCK) s_hist[i] = 0;
+    if (tid < 2) s_desc[tid] = 0;
 
     // Zero this sort's lookback words (grid-stride, 16-byte stores).
     {
@@ -198,12 +215,20 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
     const uint32_t begin = min(seg_end, seg_begin + (blockIdx.x % a.chunks_per_seg) * a.chunk);
     const uint32_t end = min(seg_end, begin + a.chunk);
     uint32_t* const hist = a.hist + seg * (MGPU_MAX_PASSES * RADIX);
-    // Early-exit check: descents of the sort's bit field between neighbours
-    // (each key against the one after it, across chunk ends too).
+    // Early-exit check: descents between neighbours (each key against the one
+    // after it, across chunk ends too) of the sort's bit field and of the
+    // first pass's digit (what the reference's per-pass count checks: the
+    // whole key and the current digit).  The key after the last of a lane's
+    // vector is the first of the next lane's vector (a shuffle), read from
+    // memory only for the last lane and at the end of the vectors.
     const bool check = a.unsorted != nullptr;
     const uint32_t cs = a.check_shift, cm = a.check_mask;
-    uint32_t descents = 0;
+    uint32_t descents = 0, first_desc = 0;
     auto bf = [&](uint32_t k) { return (k >> cs) & cm; };
+    auto pair = [&](uint32_t x, uint32_t y) {  // x just before y in the input
+        descents += bf(x) > bf(y);
+        first_desc += ((x >> a.shift[0]) & a.mask[0]) > ((y >> a.shift[0]) & a.mask[0]);
+    };
     if (begin < end) {
         if (a.vec_ok) {
             typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -239,29 +264,52 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                         }
                     }
                     if (check) {
-                        descents += (bf(q[u].x) > bf(q[u].y)) + (bf(q[u].y) > bf(q[u].z)) +
-                                    (bf(q[u].z) > bf(q[u].w));
+                        pair(q[u].x, q[u].y);
+                        pair(q[u].y, q[u].z);
+                        pair(q[u].z, q[u].w);
+                        // Vector i + 1 is in the next lane when that lane
+                        // holds one of this chunk's vectors (it is active).
                         const uint32_t next = begin + 4 * i + 4;
-                        if (next < a.n) descents += bf(q[u].w) > bf(a.keys[next]);
+                        const uint32_t lane = tid & 63;
+                        const uint32_t nb = __shfl_down(q[u].x, 1, 64);
+                        if (lane < 63 && i + 1 < nvec) pair(q[u].w, nb);
+                        else if (next < a.n) pair(q[u].w, a.keys[next]);
                     }
                 }
             }
             for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) {
                 const uint32_t k = a.keys[i];
                 count_key(k);
-                if (check && i + 1 < a.n) descents += bf(k) > bf(a.keys[i + 1]);
+                if (check && i + 1 < a.n) pair(k, a.keys[i + 1]);
             }
         } else {
             for (uint32_t i = begin + tid; i < end; i += BLOCK) {
                 const uint32_t k = a.keys[i];
                 count_key(k);
-                if (check && i + 1 < a.n) descents += bf(k) > bf(a.keys[i + 1]);
+                if (check && i + 1 < a.n) pair(k, a.keys[i + 1]);
             }
         }
     }
-    if (check && descents)
-        __hip_atomic_fetch_add(a.unsorted, descents, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (check) {
+        // Sums over the wave, then over the workgroup in LDS: one global
+        // atomic per counter and workgroup.
+        auto wave_sum = [](uint32_t v) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            return v;
+        };
+        const uint32_t dsum = wave_sum(descents), fsum = wave_sum(first_desc);
+        if ((tid & 63) == 0) {
+            if (dsum) __hip_atomic_fetch_add(&s_desc[0], dsum, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (fsum) __hip_atomic_fetch_add(&s_desc[1], fsum, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
     __syncthreads();
+    if (check && tid < 2 && s_desc[tid])
+        __hip_atomic_fetch_add(a.unsorted + tid, s_desc[tid], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
 
     if constexpr (COLS == 0) {
         for (uint32_t i = tid; i < (uint32_t)(MGPU_MAX_PASSES * RADIX); i += BLOCK) {
@@ -394,6 +442,82 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
     }
 }
 
+// The same over whole arrays (or 64K-key pieces of larger ones), one
+// 1024-thread workgroup each, with lane-column histograms: per (pass, digit
+// pair) a row of 32 words, word c counting the keys of the lanes with
+// lane % 32 == c in two 16-bit halves (even digit low, odd digit high).  The
+// 32 lanes of a half-wave then always add to distinct banks, and the lanes
+// that share a word (lane and lane + 32 of a wave) are at most two, where the
+// per-wave histograms had same-digit lanes serialize on one word.  A column
+// counts at most 32 threads x 64 keys of a 64K-key piece: 16 bits suffice.
+__device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
+    constexpr int BLOCK = 1024;
+    constexpr uint32_t COLS = 32, ROWS = MGPU_MAX_PASSES * RADIX / 2;
+    __shared__ uint32_t s_hist[ROWS * COLS];  // 64 KB
+    const uint32_t tid = threadIdx.x, col = tid & (COLS - 1);
+    for (uint32_t i = tid; i < ROWS * COLS; i += BLOCK) s_hist[i] = 0;
+    {
+        uint4* c = reinterpret_cast<uint4*>(a.clear);
+        const uint64_t nvec = a.clear_words >> 2;
+        for (uint64_t i = (uint64_t)blockIdx.x * BLOCK + tid; i < nvec;
+             i += (uint64_t)gridDim.x * BLOCK)
+            c[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (blockIdx.x >= a.num_chunks) return;  // clearing only (uniform per workgroup)
+    __syncthreads();
+    const uint32_t* ch = a.chunks + 4 * blockIdx.x;
+    const uint32_t sg = ch[0], begin = ch[1], end = ch[2];
+    const uint32_t* keys = a.segs[sg].keys[0];
+    const uint32_t np = a.num_passes;
+    auto add = [&](uint32_t p, uint32_t k) {
+        const uint32_t d = (k >> a.shift[p]) & a.mask[p];
+        __hip_atomic_fetch_add(&s_hist[((p * RADIX + d) >> 1) * COLS + col], 1u << ((d & 1u) << 4),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    };
+    auto count_key = [&](uint32_t k) {
+#pragma unroll
+        for (int p = 0; p < MGPU_MAX_PASSES; ++p)
+            if ((uint32_t)p < np) add(p, k);
+    };
+    // Scalar head up to 16-byte alignment, vector body, scalar tail.
+    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(keys + begin) >> 2) & 3);
+    const uint32_t vbeg = min(end, begin + ((4 - mis) & 3));
+    const uint32_t nvec = (end - vbeg) >> 2;
+    for (uint32_t i = begin + tid; i < vbeg; i += BLOCK) count_key(keys[i]);
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4* v = reinterpret_cast<const u32x4*>(keys + vbeg);
+    constexpr int UNROLL = 16;  // a 64K-key piece: every vector of a thread in flight
+    for (uint32_t i0 = tid; i0 < nvec; i0 += UNROLL * BLOCK) {
+        u32x4 q[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u)
+            if (i0 + u * BLOCK < nvec) q[u] = __builtin_nontemporal_load(v + i0 + u * BLOCK);
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            if (i0 + u * BLOCK >= nvec) break;
+            count_key(q[u].x);
+            count_key(q[u].y);
+            count_key(q[u].z);
+            count_key(q[u].w);
+        }
+    }
+    for (uint32_t i = vbeg + 4 * nvec + tid; i < end; i += BLOCK) count_key(keys[i]);
+    __syncthreads();
+    // Thread (p, d) = (tid / 256, tid % 256) sums its digit's 32 columns,
+    // starting at column tid so that a half-wave reads 32 distinct banks.
+    const uint32_t p = tid / RADIX, d = tid % RADIX;
+    if (p < np) {
+        const uint32_t row = (p * RADIX + d) >> 1, sh = (d & 1u) << 4;
+        uint32_t sum = 0;
+#pragma unroll
+        for (uint32_t c = 0; c < COLS; ++c)
+            sum += (s_hist[row * COLS + ((c + tid) & (COLS - 1))] >> sh) & 0xFFFFu;
+        if (sum)
+            __hip_atomic_fetch_add(&a.hist[((size_t)sg * MGPU_MAX_PASSES + p) * RADIX + d], sum,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 __device__ void multi_scan_body(const mgpuMultiScanArgs& a) {
     __shared__ uint32_t s_tmp[4];
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -456,11 +580,21 @@ __device__ __forceinline__ void lookback_read(const uint32_t* lb, int32_t t0, ui
 
 // Returns the exclusive prefix of the tile for digit d.  `w` holds the first
 // round's words, read by the caller with lookback_read(lb, tile - 1, ...).
+// A word still unpublished after spin_limit polls (about a second at the
+// default; only a broken tile counter could make a wait that long) ends the
+// wait with what has been summed so far and sets error code 1.  spin_limit 0
+// ends every wait at once, before anything is summed (tests of the error
+// path: every tile after a chain's first one gives up).
 template <uint32_t G, uint32_t K, bool LOCAL>
 __device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t tile, uint32_t d,
                                                     uint32_t k, uint32_t lane,
-                                                    uint32_t* error_flag, uint32_t (&w)[K]) {
+                                                    uint32_t* error_flag, uint32_t spin_limit,
+                                                    uint32_t (&w)[K]) {
     if (tile == 0) return 0;
+    if (spin_limit == 0) {
+        set_error(error_flag, 1u);
+        return 0;
+    }
     constexpr uint32_t GM = (1u << G) - 1u;
     const uint32_t gfirst = lane & ~(G - 1u);  // the group's first lane in the wave
     uint32_t prefix = 0, spins = 0;
@@ -482,8 +616,8 @@ __device__ __forceinline__ uint32_t lookback_window(const uint32_t* lb, uint32_t
         // Lanes whose words count: up to the first with a prefix, or all.
         const uint32_t upto = pref ? ((pref & (0u - pref)) << 1) - 1u : GM;
         if ((rdy & upto) != upto) {
-            if (++spins > kSpinLimit) {
-                st_agent(error_flag, 1u);
+            if (++spins > spin_limit) {
+                set_error(error_flag, 1u);
                 return prefix;
             }
             __builtin_amdgcn_s_sleep(1);
@@ -605,11 +739,22 @@ __device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t n, ui
 // instruction that hit the same count are served in ascending lane order,
 // which the engine checks at creation (mgpu_rank_probe) before it picks
 // these kernels.
-// XSEG (single array only): the workgroup takes tiles of its XCD's segment
-// until the segment is done; the segment has a lookback chain of its own.
-template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool XSEG, typename Args>
+// XSEG: the workgroup takes tiles of its XCD's share until it is done.  A
+// single array is cut into one segment of tiles per XCD with a lookback chain
+// of its own (each pass counting the next one's per-segment digit starts);
+// with MULTI, whole arrays are assigned to XCDs, so each array's chain stays
+// on one XCD and nothing else changes.
+// EE (early exit, RANK 1 only): a wave whose keys are already in digit order
+// ranks them by position instead of by LDS atomics (the role of the
+// reference's sorted-block path, sort.cu:135-158, with the check made here
+// per wave instead of per block in a count kernel).  Its keys then stage to
+// consecutive LDS words, free of bank conflicts; a tile whose waves all are in
+// order, and in order across waves, stages every key to its own place.
+template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool XSEG, bool EE, typename Args>
 __device__ void rank_scatter_body(const Args& a) {
-    static_assert(!(MULTI && XSEG), "segments per XCD are for single arrays");
+    static_assert(!EE || RANK == 1, "early exit builds on the atomic-add ranking");
+    static_assert(!(EE && MULTI), "early exit is for single arrays");
+    constexpr bool XSINGLE = XSEG && !MULTI;  // one array cut into segments per XCD
     static_assert(BLOCK % 64 == 0 && BLOCK >= (int)RADIX, "block must cover the radix");
     constexpr int W = BLOCK / 64;
     constexpr uint32_t TILE = BLOCK * IPT;
@@ -635,9 +780,9 @@ __device__ void rank_scatter_body(const Args& a) {
     // XSEG: this segment's digit starts, and the next pass's digit counts of
     // the keys this workgroup writes, by destination segment.
     constexpr uint32_t NSEG = MGPU_XSEG_COUNT_SEGS;
-    __shared__ uint32_t s_dstart[XSEG ? RADIX : 1];
-    __shared__ uint32_t s_next[XSEG ? NSEG * RADIX : 1];
-    __shared__ uint32_t s_spare[XSEG ? 64 : 1];
+    __shared__ uint32_t s_dstart[XSINGLE ? RADIX : 1];
+    __shared__ uint32_t s_next[XSINGLE ? NSEG * RADIX : 1];
+    __shared__ uint32_t s_spare[XSINGLE ? 64 : 1];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
@@ -651,7 +796,14 @@ __device__ void rank_scatter_body(const Args& a) {
         uint32_t xcc;
         asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
         seg = xcc & 0xFu;
-        if (seg < a.num_segs) {
+        if constexpr (MULTI) {
+            if (seg < a.num_xcds) {
+                seg_first = a.xcd_tile_begin[seg];
+                seg_tiles = a.xcd_tile_begin[seg + 1] - seg_first;
+            } else {
+                seg_tiles = 0;
+            }
+        } else if (seg < a.num_segs) {
             seg_first = mgpu_seg_first_tile(seg, a.num_tiles, a.num_segs);
             seg_tiles = mgpu_seg_first_tile(seg + 1, a.num_tiles, a.num_segs) - seg_first;
         } else {
@@ -674,7 +826,7 @@ __device__ void rank_scatter_body(const Args& a) {
     const float seg_half = 0.5f * seg_rcp;
     uint32_t next_mode = RADIX;  // most frequent digit of the next pass
     uint32_t next_width = 0;
-    if constexpr (XSEG) {
+    if constexpr (XSINGLE) {
         count_next = a.seg_count_out != nullptr;
         if (count_next) {
             next_mode = *a.next_mode;
@@ -702,7 +854,7 @@ __device__ void rank_scatter_body(const Args& a) {
                 t = __hip_atomic_fetch_add(a.tile_counter + seg, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
             s_tile[0] = t;
-            s_tile[1] = !XSEG && t < a.num_tiles ? find_array<MULTI>(a, t) : 0u;
+            s_tile[1] = MULTI && t < seg_tiles ? find_array<MULTI>(a, seg_first + t) : 0u;
         }
         __syncthreads();
         // Tile and array indices are the same for every lane: keep them (and all
@@ -711,11 +863,14 @@ __device__ void rank_scatter_body(const Args& a) {
         if (gtile >= seg_tiles) {
             // One tile per workgroup: only a counter left dirty gets here.  A
             // segment's workgroup: the segment is done.
-            if (!XSEG && tid == 0) st_agent(a.error_flag, 2u);
+            if (!XSEG && tid == 0) set_error(a.error_flag, 2u);
             break;
         }
-        TileJob job = make_job<MULTI>(a, gtile, __builtin_amdgcn_readfirstlane(s_tile[1]));
-        if constexpr (XSEG) {
+        // MULTI: tiles are numbered across the arrays; XSINGLE: within the
+        // segment, whose chain starts at its first tile.
+        TileJob job = make_job<MULTI>(a, MULTI ? seg_first + gtile : gtile,
+                                      __builtin_amdgcn_readfirstlane(s_tile[1]));
+        if constexpr (XSINGLE) {
             job.first = seg_first;
             job.lookback += (size_t)seg_first * RADIX;
         }
@@ -744,6 +899,47 @@ __device__ void rank_scatter_body(const Args& a) {
 
         // Rank within the wave: pos[j] = earlier same-digit keys of this wave.
         uint32_t pos[IPT];
+        bool wave_sorted = false;
+        if constexpr (EE) {
+            // Key q = j * 64 + lane of the wave against key q - 1.
+            uint64_t desc = 0;
+            uint32_t tail = 0;  // digit of the previous slot's lane 63 (none before slot 0)
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t d = digit_of(j);
+                desc |= __builtin_amdgcn_ballot_w64(prev_lane(d, tail) > d);
+                tail = __builtin_amdgcn_readlane(d, 63);
+            }
+            wave_sorted = desc == 0;
+        }
+        if (EE && wave_sorted) {
+            // Runs of equal digits in key order: a key's rank is its distance
+            // from its run's first key, and a run's length is the digit's
+            // count.  The run starts live in the staging area, unused until
+            // after the next barrier.
+            uint32_t* const first = s_stage + wave * RADIX;
+            uint32_t tail = RADIX;  // no digit: key 0 starts a run
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t d = digit_of(j);
+                if (prev_lane(d, tail) != d) first[d] = (uint32_t)j * 64 + lane;
+                tail = __builtin_amdgcn_readlane(d, 63);
+            }
+            wave_sync();
+            tail = RADIX;
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t d = digit_of(j), q = (uint32_t)j * 64 + lane;
+                pos[j] = q - first[d];
+                const uint32_t pd = prev_lane(d, tail);
+                if (q != 0 && pd != d) s_count[wave][pd] = q - first[pd];  // run of pd ends at q
+                tail = __builtin_amdgcn_readlane(d, 63);
+            }
+            if (lane == 63) {
+                const uint32_t d = digit_of(IPT - 1);
+                s_count[wave][d] = WAVE_TILE - first[d];
+            }
+        } else {
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const uint32_t d = digit_of(j);
@@ -781,6 +977,7 @@ __device__ void rank_scatter_body(const Args& a) {
                 }
             }
         }
+        }  // ranking
         __syncthreads();
 
         // Per digit (thread d): wave-exclusive offsets and the tile total,
@@ -833,13 +1030,13 @@ __device__ void rank_scatter_body(const Args& a) {
         }
 
         {
-            const uint32_t prefix = lookback_window<G, K, XSEG>(job.lookback, tile, lb_digit, lb_k,
-                                                                lane, a.error_flag, lb_words);
+            const uint32_t prefix = lookback_window<G, K, XSEG>(
+                job.lookback, tile, lb_digit, lb_k, lane, a.error_flag, a.spin_limit, lb_words);
             if (lb_k == 0) {
                 if (tile > 0)
                     lookback_publish<XSEG>(&job.lookback[tile * RADIX + lb_digit],
                                            MGPU_LB_PREFIX | (prefix + s_tcount[lb_digit]));
-                const uint32_t dstart = XSEG ? s_dstart[lb_digit] : job.digit_start[lb_digit];
+                const uint32_t dstart = XSINGLE ? s_dstart[lb_digit] : job.digit_start[lb_digit];
                 s_global[lb_digit] = dstart + prefix - s_lstart[lb_digit];
             }
         }
@@ -878,11 +1075,11 @@ __device__ void rank_scatter_body(const Args& a) {
             if constexpr (JOINT)
                 __builtin_amdgcn_raw_buffer_store_b32(vout[i], rv, (int)(dest[i] * 4u), 0, 0);
         }
-        if (bad) st_agent(a.error_flag, 4u);
+        if (bad) set_error(a.error_flag, 4u);
         // Segments per XCD: the next pass's digit of every key written, by
         // the segment of the next pass that its position falls in.
         // (Computing the segment costs less than a per-digit LDS lookup.)
-        if constexpr (XSEG) {
+        if constexpr (XSINGLE) {
             if (count_next) {
 #pragma unroll
                 for (int i = 0; i < IPT; ++i) {
@@ -902,14 +1099,14 @@ __device__ void rank_scatter_body(const Args& a) {
         if constexpr (VMODE == MGPU_VALUES_MULTI) {
             for (uint32_t v = 0; v < a.num_values; ++v) {
                 load_striped<IPT>(job_vals_in<MULTI>(a, job, v), n, wave_base, lane, val);
+                uint32_t* dstv = job_vals_out<MULTI>(a, job, v);
                 __syncthreads();
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
                 __syncthreads();
-                uint32_t* dstv = job_vals_out<MULTI>(a, job, v);
 #pragma unroll
                 for (int i = 0; i < IPT; ++i)
-                    if ((full || i * BLOCK < vrem) && dest[i] < n) dstv[dest[i]] = s_stage[i * BLOCK + tid];
+                    if (dest[i] < n) dstv[dest[i]] = s_stage[i * BLOCK + tid];
             }
         }
 
@@ -925,7 +1122,7 @@ __device__ void rank_scatter_body(const Args& a) {
     // The next pass's counts of this workgroup's keys, and the last
     // workgroup to finish checks that every segment was taken to its end
     // (each needs a workgroup on its XCD).
-    if constexpr (XSEG) {
+    if constexpr (XSINGLE) {
         if (count_next) {
             __syncthreads();
             for (uint32_t i = tid; i < NSEG * RADIX; i += BLOCK) {
@@ -934,14 +1131,21 @@ __device__ void rank_scatter_body(const Args& a) {
                                               __HIP_MEMORY_SCOPE_AGENT);
             }
         }
+    }
+    if constexpr (XSEG) {
         if (tid == 0) {
             const uint32_t done =
                 __hip_atomic_fetch_add(a.finished, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (done + 1 == gridDim.x) {
-                for (uint32_t x = 0; x < a.num_segs; ++x) {
-                    const uint32_t tiles = mgpu_seg_first_tile(x + 1, a.num_tiles, a.num_segs) -
-                                           mgpu_seg_first_tile(x, a.num_tiles, a.num_segs);
-                    if (ld_agent(a.tile_counter + x) < tiles) st_agent(a.error_flag, 5u);
+                uint32_t segs;
+                if constexpr (MULTI) segs = a.num_xcds;
+                else segs = a.num_segs;
+                for (uint32_t x = 0; x < segs; ++x) {
+                    uint32_t tiles;
+                    if constexpr (MULTI) tiles = a.xcd_tile_begin[x + 1] - a.xcd_tile_begin[x];
+                    else tiles = mgpu_seg_first_tile(x + 1, a.num_tiles, a.num_segs) -
+                                 mgpu_seg_first_tile(x, a.num_tiles, a.num_segs);
+                    if (ld_agent(a.tile_counter + x) < tiles) set_error(a.error_flag, 5u);
                 }
             }
         }
@@ -997,6 +1201,10 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_count(mgpuMultiCoun
     multi_count_body<256>(a);
 }
 
+extern "C" __global__ void __launch_bounds__(1024) mgpu_multi_count_cols(mgpuMultiCountArgs a) {
+    multi_count_cols_body(a);
+}
+
 extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanArgs a) {
     multi_scan_body(a);
 }
@@ -1004,16 +1212,35 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
 #define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
-        rank_scatter_body<B, I, R, MODE, false, false>(a);                                    \
+        rank_scatter_body<B, I, R, MODE, false, false, false>(a);                             \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_x_v##V(mgpuPassArgs a) {                       \
-        rank_scatter_body<B, I, R, MODE, false, true>(a);                                     \
+        rank_scatter_body<B, I, R, MODE, false, true, false>(a);                              \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
-        rank_scatter_body<B, I, R, MODE, true, false>(a);                                     \
+        rank_scatter_body<B, I, R, MODE, true, false, false>(a);                              \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_multi_scatter_b##B##_i##I##_r##R##_x_v##V(mgpuMultiPassArgs a) {                 \
+        rank_scatter_body<B, I, R, MODE, true, true, false>(a);                               \
     }
+// Early-exit forms (atomic-add ranking only): one chain, and a chain per XCD.
+#define MGPU_RANK_SCATTER_EE_MODE(B, I, V, MODE)                                              \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r1_ee_v##V(mgpuPassArgs a) {                          \
+        rank_scatter_body<B, I, 1, MODE, false, false, true>(a);                              \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r1_x_ee_v##V(mgpuPassArgs a) {                        \
+        rank_scatter_body<B, I, 1, MODE, false, true, true>(a);                               \
+    }
+#define MGPU_RANK_SCATTER_EE(B, I)                                                            \
+    MGPU_RANK_SCATTER_EE_MODE(B, I, 0, MGPU_VALUES_NONE)                                      \
+    MGPU_RANK_SCATTER_EE_MODE(B, I, 1, MGPU_VALUES_INDEX)                                     \
+    MGPU_RANK_SCATTER_EE_MODE(B, I, 2, MGPU_VALUES_SINGLE)                                    \
+    MGPU_RANK_SCATTER_EE_MODE(B, I, 3, MGPU_VALUES_MULTI)
 #define MGPU_RANK_SCATTER(B, I, R)                                                            \
     MGPU_RANK_SCATTER_MODE(B, I, R, 0, MGPU_VALUES_NONE)                                      \
     MGPU_RANK_SCATTER_MODE(B, I, R, 1, MGPU_VALUES_INDEX)                                     \
@@ -1030,6 +1257,8 @@ MGPU_RANK_SCATTER(1024, 16, 0)
 MGPU_RANK_SCATTER(256, 16, 1)
 MGPU_RANK_SCATTER(512, 16, 1)
 MGPU_RANK_SCATTER(1024, 16, 1)
+MGPU_RANK_SCATTER_EE(512, 16)
+MGPU_RANK_SCATTER_EE(1024, 16)
 
 // Check of the ordering RANK 1 relies on: wave w of one 1024-thread
 // workgroup adds 1 per lane, for 16 rounds, to the count of digit

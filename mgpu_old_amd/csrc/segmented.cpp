@@ -20,35 +20,43 @@
 
 namespace {
 
-// Segments of at least this size go to the device-wide passes (all of them
-// together, each spread over many workgroups with a lookback chain) instead
-// of one workgroup each.  Above 4096 keys (the largest a workgroup sorts in
-// LDS) one workgroup would stream the segment through global memory tile by
-// tile, bound by memory latency: a 64K-key segment took up to 1.3 ms, and a
-// batch of a few such segments left the GPU idle.  Measured: 4096 segments
-// log-uniform in [1, 2^16] in 2.04 ms with the threshold above 2^16, 0.74 ms
-// above 4096.  MGPUSORT_SEG_WIDE_MIN overrides it (measurements).
-constexpr uint32_t kDeviceWideMin = 4096 + 1;
-uint32_t device_wide_min() {
+// Segments of at least wide.minKeys keys go to the device-wide passes (all
+// of them together, each spread over many workgroups with a lookback chain)
+// instead of one workgroup each.  Above 4096 keys (the largest a 256-thread
+// workgroup sorts in LDS) one workgroup would stream the segment through
+// global memory tile by tile, bound by memory latency: a 64K-key segment took
+// up to 1.3 ms, and a batch of a few such segments left the GPU idle.
+// Measured: 4096 segments log-uniform in [1, 2^16] in 2.04 ms with the
+// threshold above 2^16, 0.74 ms above 4096.  The tuning table's "wide" row
+// sets it; MGPUSORT_SEG_WIDE_MIN overrides both (measurements).
+uint32_t device_wide_min(const sortEngine_d* e) {
     const char* v = std::getenv("MGPUSORT_SEG_WIDE_MIN");
     const long x = v ? std::strtol(v, nullptr, 10) : 0;
-    return x > 0 ? (uint32_t)x : kDeviceWideMin;
+    return x > 0 ? (uint32_t)x : e->segWide.minKeys;
 }
-// Keys per digit_count workgroup on that path (MGPUSORT_SEG_WIDE_CHUNK).
-constexpr uint32_t kWideChunk = 1u << 14;
-uint32_t wide_chunk() {
+// Keys per digit_count workgroup on that path (a multiple of 1024 up to 2^20,
+// where the 16-bit lane-column counts of the count kernel would start to
+// overflow; MGPUSORT_SEG_WIDE_CHUNK overrides the table): at the default
+// 2^16 a segment of up to 2^16 keys is one piece, so its histogram is merged
+// into global memory once.
+uint32_t clamp_chunk(long x) { return (uint32_t)std::min<long>(std::max<long>(x, 1024), 1L << 20) & ~1023u; }
+uint32_t wide_chunk(const sortEngine_d* e) {
     const char* v = std::getenv("MGPUSORT_SEG_WIDE_CHUNK");
     const long x = v ? std::strtol(v, nullptr, 10) : 0;
-    return x >= 1024 ? (uint32_t)x & ~1023u : kWideChunk;
+    return x >= 1024 ? clamp_chunk(x) : e->segWide.chunk;
 }
 
 const SegTuneEntry kDefaultTable[] = {
-    // max log2(n), radix bits, items per thread
-    {6, 8, 8}, {10, 8, 16}, {12, 8, 16}, {31, 8, 16},
+    // max log2(n), radix bits, items per thread, workgroup size
+    {6, 8, 8, 256}, {10, 8, 16, 256}, {12, 8, 16, 256}, {31, 8, 16, 256},
 };
 
-bool parse_variant(const char* s, int* rb, int* ipt) {
-    return s && std::sscanf(s, "r%di%d", rb, ipt) == 2;
+// "r<bits>i<items>" (256 threads) or "r<bits>i<items>b<threads>".
+bool parse_variant(const char* s, int* rb, int* ipt, int* block) {
+    if (!s) return false;
+    *block = 256;
+    const int got = std::sscanf(s, "r%di%db%d", rb, ipt, block);
+    return got >= 2 && seg_variant_valid(*rb, *ipt, *block);
 }
 
 }  // namespace
@@ -59,26 +67,30 @@ sortStatus_t mgpu_seg_load(sortEngine_d* e) {
         e->segModule = nullptr;
         return SORT_STATUS_KERNEL_NOT_FOUND;
     }
-    for (int rb = kSegMinBits; rb <= kSegMaxBits; ++rb)
-        for (int k = 0; k < 2; ++k) {
-            int ipt = k ? 16 : 8;
-            char name[64];
-            std::snprintf(name, sizeof(name), "mgpu_segsort_r%d_i%d", rb, ipt);
-            hipFunction_t f = nullptr;
-            if (hipModuleGetFunction(&f, e->segModule, name) != hipSuccess)
-                return SORT_STATUS_KERNEL_ERROR;
-            e->kernelNames[f] = name;
-            int blocks = 0;
-            if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, 256, 0) !=
-                    hipSuccess || blocks < 1)
-                blocks = 1;
-            e->segFn[rb - kSegMinBits][k] = f;
-            e->segOccupancy[rb - kSegMinBits][k] = blocks;
-        }
+    for (int v = 0; v < kSegVariants; ++v) {
+        const int rb = seg_variant_bits(v), ipt = seg_variant_ipt(v), block = seg_variant_block(v);
+        if (!seg_variant_valid(rb, ipt, block)) continue;
+        char name[64];
+        std::snprintf(name, sizeof(name), "mgpu_segsort_r%d_i%d_b%d", rb, ipt, block);
+        hipFunction_t f = nullptr;
+        if (hipModuleGetFunction(&f, e->segModule, name) != hipSuccess)
+            return SORT_STATUS_KERNEL_ERROR;
+        e->kernelNames[f] = name;
+        int blocks = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, 0) !=
+                hipSuccess || blocks < 1)
+            blocks = 1;
+        e->segFn[v] = f;
+        e->segOccupancy[v] = blocks;
+    }
 
     // Tuning table: kernelPath + "mgpusort_seg_tuning.txt" when present
-    // (written by the autotuner), else the built-in default.
+    // (written by the autotuner), else the built-in default.  Rows
+    // "<max ceil(log2 n)> <radix bits> <keys per thread> [<threads>]" pick the
+    // persistent kernel per size bucket; a row "wide <min keys> <chunk>
+    // <auto|small|large>" sets the device-wide passes.
     e->segTable.assign(std::begin(kDefaultTable), std::end(kDefaultTable));
+    e->segWide = SegWideTune();
     std::ifstream f(e->kernelPath + "mgpusort_seg_tuning.txt");
     if (f) {
         std::vector<SegTuneEntry> t;
@@ -86,11 +98,21 @@ sortStatus_t mgpu_seg_load(sortEngine_d* e) {
         while (std::getline(f, line)) {
             if (line.empty() || line[0] == '#') continue;
             std::istringstream is(line);
+            if (line.compare(0, 5, "wide ") == 0) {
+                std::string w, tile;
+                long minKeys = 0, chunk = 0;
+                if (is >> w >> minKeys >> chunk >> tile && minKeys > 0 && chunk >= 1024 &&
+                    (tile == "auto" || tile == "small" || tile == "large")) {
+                    e->segWide.minKeys = (uint32_t)minKeys;
+                    e->segWide.chunk = clamp_chunk(chunk);
+                    e->segWide.tile = tile == "small" ? 1 : tile == "large" ? 2 : 0;
+                }
+                continue;
+            }
             SegTuneEntry en;
             if (is >> en.maxLog2 >> en.radixBits >> en.itemsPerThread) {
-                if (en.radixBits < kSegMinBits || en.radixBits > kSegMaxBits ||
-                    (en.itemsPerThread != 8 && en.itemsPerThread != 16))
-                    continue;
+                if (!(is >> en.blockThreads)) en.blockThreads = 256;
+                if (!seg_variant_valid(en.radixBits, en.itemsPerThread, en.blockThreads)) continue;
                 t.push_back(en);
             }
         }
@@ -118,20 +140,20 @@ int ceil_log2(uint32_t n) {
     return l;
 }
 
-void pick_variant(const sortEngine_d* e, uint32_t n, int* rb, int* ipt) {
+void pick_variant(const sortEngine_d* e, uint32_t n, int* rb, int* ipt, int* block) {
     const char* forced = std::getenv("MGPUSORT_SEG_VARIANT");
-    if (parse_variant(forced, rb, ipt) && *rb >= kSegMinBits && *rb <= kSegMaxBits &&
-        (*ipt == 8 || *ipt == 16))
-        return;
+    if (parse_variant(forced, rb, ipt, block)) return;
     int l = ceil_log2(n);
     for (const SegTuneEntry& t : e->segTable)
         if (l <= t.maxLog2) {
             *rb = t.radixBits;
             *ipt = t.itemsPerThread;
+            *block = t.blockThreads;
             return;
         }
     *rb = 8;
     *ipt = 16;
+    *block = 256;
 }
 
 }  // namespace
@@ -154,11 +176,11 @@ sortStatus_t mgpu_lane_ensure_side(SortLane& lane) {
 size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys, uint32_t minTile) {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t segs = std::max<size_t>(1, numSegments);
-    const size_t chunks = keys / wide_chunk() + segs;
+    const size_t chunks = keys / 1024 + segs;  // pieces of at least 1024 keys
     const size_t tiles = keys / std::max(1u, minTile) + segs;
     return 2 * al(sizeof(mgpuSegDesc) * segs) + al(sizeof(uint32_t) * segs) +
            al(sizeof(uint32_t) * (segs + 1)) + al(4 * sizeof(uint32_t) * chunks) +
-           al(sizeof(uint32_t) * tiles);
+           al(sizeof(uint32_t) * tiles) + al(sizeof(uint32_t) * (MGPU_MAX_SEGS + 1));
 }
 
 sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterWords,
@@ -194,7 +216,7 @@ sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterW
 
 sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
                             int numSegments, int valueCount, int firstBit, int endBit,
-                            SegPlanData* out, SegArena* arena) {
+                            SegPlanData* out, SegArena* arena, bool ownScratch) {
     SegPlanData& P = *out;
     P.engine = e;
     P.valueCount = valueCount;
@@ -205,7 +227,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     P.hostSegs.assign(segs, segs + numSegments);
 
     // Group segments by kernel variant; very large ones go device-wide.
-    const uint32_t wideMin = device_wide_min(), chunkKeys = wide_chunk();
+    const uint32_t wideMin = device_wide_min(e), chunkKeys = wide_chunk(e);
     struct Group {
         std::vector<uint32_t> big, small;
     };
@@ -217,9 +239,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             P.wide.push_back((uint32_t)i);
             continue;
         }
-        int rb, ipt;
-        pick_variant(e, n, &rb, &ipt);
-        Group& g = groups[seg_variant_index(rb, ipt)];
+        int rb, ipt, block;
+        pick_variant(e, n, &rb, &ipt, &block);
+        Group& g = groups[seg_variant_index(rb, ipt, block)];
         if (n <= 64u * (uint32_t)ipt) g.small.push_back((uint32_t)i);
         else g.big.push_back((uint32_t)i);
     }
@@ -240,10 +262,12 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         lists.insert(lists.end(), g.small.begin(), g.small.end());
         L.numBig = (uint32_t)g.big.size();
         L.numSmall = (uint32_t)g.small.size();
-        const int rb = seg_variant_bits(v), ipt = seg_variant_ipt(v);
-        L.fn = e->segFn[rb - kSegMinBits][ipt == 16];
-        const uint32_t work = L.numBig + (L.numSmall + 3) / 4;
-        const uint32_t maxGrid = (uint32_t)(e->segOccupancy[rb - kSegMinBits][ipt == 16] * e->numCUs);
+        const int rb = seg_variant_bits(v);
+        L.fn = e->segFn[v];
+        L.block = (uint32_t)seg_variant_block(v);
+        const uint32_t wavesPerGroup = L.block / 64;
+        const uint32_t work = L.numBig + (L.numSmall + wavesPerGroup - 1) / wavesPerGroup;
+        const uint32_t maxGrid = (uint32_t)(e->segOccupancy[v] * e->numCUs);
         L.grid = std::max(1u, std::min(work, maxGrid));
         L.passes = mgpu_plan_passes(P.numBits, rb, L.widths);
         P.launches.push_back(L);
@@ -260,8 +284,48 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         // take the small tile shape (4096 segments log-uniform in [1, 2^16]:
         // 0.74 -> 0.69 ms); the large shape once they average 4 large tiles.
         const uint64_t avgKeys = wideKeys / P.wide.size();
-        P.wideVariant = avgKeys >= 4ull * e->large.tile() ? &mgpu_pick_variant(e, wideKeys) : &e->small;
+        if (e->segWide.tile == 1) P.wideVariant = &e->small;
+        else if (e->segWide.tile == 2) P.wideVariant = &e->large;
+        else P.wideVariant = avgKeys >= 4ull * e->large.tile() ? &mgpu_pick_variant(e, wideKeys) : &e->small;
         const uint32_t tile = P.wideVariant->tile();
+        // Lookback chains per XCD: each segment's tiles on one XCD, whose L2
+        // alone holds its chain (~0.45 us a round trip instead of ~0.85 us
+        // through the device-coherent level).  Segments go to the XCD with
+        // the fewest tiles so far, largest first, and are then laid out XCD
+        // by XCD; only when no segment outweighs a quarter of an XCD's share
+        // (else one XCD would do most of the work, and one chain across the
+        // device is kept).
+        P.wideXcds = 1;
+        P.xcdTileBegin.clear();
+        const uint32_t X = e->xcdSegs;
+        if (X > 1 && P.wideVariant->multiX[MGPU_VALUES_NONE]) {
+            uint64_t total = 0, most = 0;
+            for (uint32_t k : P.wide) {
+                const uint64_t t = (segs[k].n + tile - 1) / tile;
+                total += t;
+                most = std::max(most, t);
+            }
+            if (most * 4 * X <= total) {
+                std::vector<uint32_t> order(P.wide);
+                std::stable_sort(order.begin(), order.end(),
+                                 [&](uint32_t a, uint32_t b) { return segs[a].n > segs[b].n; });
+                std::vector<std::vector<uint32_t>> group(X);
+                std::vector<uint64_t> load(X, 0);
+                for (uint32_t k : order) {
+                    const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) -
+                                                  load.begin());
+                    group[x].push_back(k);
+                    load[x] += (segs[k].n + tile - 1) / tile;
+                }
+                P.wide.clear();
+                P.xcdTileBegin.push_back(0);
+                for (uint32_t x = 0; x < X; ++x) {
+                    P.wide.insert(P.wide.end(), group[x].begin(), group[x].end());
+                    P.xcdTileBegin.push_back(P.xcdTileBegin.back() + (uint32_t)load[x]);
+                }
+                P.wideXcds = X;
+            }
+        }
         tileBegin.push_back(0);
         for (size_t k = 0; k < P.wide.size(); ++k) {
             const mgpuSegDesc& d = segs[P.wide[k]];
@@ -285,8 +349,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     const size_t tbBytes = al(sizeof(uint32_t) * tileBegin.size());
     const size_t chBytes = al(sizeof(uint32_t) * chunks.size());
     const size_t tsBytes = al(sizeof(uint32_t) * tileSeg.size());
+    const size_t xbBytes = al(sizeof(uint32_t) * P.xcdTileBegin.size());
     const size_t ctrWords = 4 * std::max<size_t>(1, P.launches.size());
-    const size_t payload = descBytes + listBytes + wideBytes + tbBytes + chBytes + tsBytes;
+    const size_t payload = descBytes + listBytes + wideBytes + tbBytes + chBytes + tsBytes + xbBytes;
     const size_t total = payload + ctrWords * sizeof(uint32_t);
     char* base = nullptr;
     if (!arena) {
@@ -313,6 +378,10 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
              hipMemcpy(base + off + wideBytes + tbBytes + chBytes, tileSeg.data(),
                        sizeof(uint32_t) * tileSeg.size(), hipMemcpyHostToDevice) != hipSuccess))
             return SORT_STATUS_DEVICE_ERROR;
+        if (!P.xcdTileBegin.empty() &&
+            hipMemcpy(base + off + wideBytes + tbBytes + chBytes + tsBytes, P.xcdTileBegin.data(),
+                      sizeof(uint32_t) * P.xcdTileBegin.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return SORT_STATUS_DEVICE_ERROR;
         // Stream-ordered before the launches on the lane's stream (a plain
         // hipMemset would run on the null stream, unordered with a
         // non-blocking lane stream).
@@ -334,6 +403,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             std::memcpy(h + off + wideBytes + tbBytes, chunks.data(), sizeof(uint32_t) * chunks.size());
             std::memcpy(h + off + wideBytes + tbBytes + chBytes, tileSeg.data(),
                         sizeof(uint32_t) * tileSeg.size());
+            if (!P.xcdTileBegin.empty())
+                std::memcpy(h + off + wideBytes + tbBytes + chBytes + tsBytes,
+                            P.xcdTileBegin.data(), sizeof(uint32_t) * P.xcdTileBegin.size());
         }
         if (hipMemcpyAsync(base, h, payload, hipMemcpyHostToDevice, lane.stream) != hipSuccess)
             return SORT_STATUS_DEVICE_ERROR;
@@ -347,13 +419,30 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
     P.devChunks = reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes + tbBytes);
     P.devTileSeg =
         reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes + tbBytes + chBytes);
+    P.devXcdTileBegin = reinterpret_cast<uint32_t*>(base + descBytes + listBytes + wideBytes +
+                                                    tbBytes + chBytes + tsBytes);
 
     // Device-wide segments: lookback words and per-array scratch before any
     // capture.
     if (!P.wide.empty()) {
-        sortStatus_t st = mgpu_ensure_lookback_words(
-            lane, (uint64_t)P.sortPasses * P.wideTiles * MGPU_RADIX);
-        if (!st) st = mgpu_ensure_multi(lane, (uint32_t)P.wide.size());
+        const uint64_t lbWords = (uint64_t)P.sortPasses * P.wideTiles * MGPU_RADIX;
+        sortStatus_t st = SORT_STATUS_SUCCESS;
+        if (ownScratch) {
+            // Histograms must start at zero (the scan leaves them zero after
+            // every run); lookback words are zeroed by the count kernel.
+            const size_t multiWords = (size_t)2 * P.wide.size() * MGPU_MAX_PASSES * MGPU_RADIX;
+            if (hipMalloc(&P.ownLookback, ((lbWords + 3) & ~uint64_t(3)) * sizeof(uint32_t)) !=
+                    hipSuccess ||
+                hipMalloc(&P.ownMulti, multiWords * sizeof(uint32_t)) != hipSuccess)
+                return SORT_STATUS_DEVICE_ALLOC_FAILED;
+            if (hipMemsetAsync(P.ownMulti, 0, multiWords * sizeof(uint32_t) / 2, lane.stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(lane.stream) != hipSuccess)
+                return SORT_STATUS_DEVICE_ERROR;
+        } else {
+            st = mgpu_ensure_lookback_words(lane, lbWords);
+            if (!st) st = mgpu_ensure_multi(lane, (uint32_t)P.wide.size());
+        }
         if (st) return st;
         if (!P.launches.empty() && !lane.noSide) st = mgpu_lane_ensure_side(lane);
         if (st) return st;
@@ -371,15 +460,18 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
     const uint32_t np = (uint32_t)P.sortPasses;
     const uint64_t lbWords = (uint64_t)np * P.wideTiles * MGPU_RADIX;
     const uint32_t numWide = (uint32_t)P.wide.size();
-    uint32_t* hist = lane.multi;
-    uint32_t* digitStart = lane.multi + (size_t)lane.multiArrays * MGPU_MAX_PASSES * MGPU_RADIX;
+    uint32_t* const lookback = P.ownLookback ? P.ownLookback : lane.lookback;
+    uint32_t* const hist = P.ownMulti ? P.ownMulti : lane.multi;
+    uint32_t* const digitStart =
+        P.ownMulti ? P.ownMulti + (size_t)numWide * MGPU_MAX_PASSES * MGPU_RADIX
+                   : lane.multi + (size_t)lane.multiArrays * MGPU_MAX_PASSES * MGPU_RADIX;
 
     mgpuMultiCountArgs ca;
     std::memset(&ca, 0, sizeof(ca));
     ca.segs = P.devWide;
     ca.chunks = P.devChunks;
     ca.hist = hist;
-    ca.clear = lane.lookback;
+    ca.clear = lookback;
     ca.clear_words = (lbWords + 3) & ~uint64_t(3);
     ca.num_chunks = P.wideChunks;
     ca.num_passes = np;
@@ -389,12 +481,14 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         ca.mask[p] = (1u << P.sortWidths[p]) - 1u;
         bit += P.sortWidths[p];
     }
+    // One 1024-thread workgroup per piece (two fit on a CU), enough of them
+    // to zero the lookback words quickly as well.
     uint32_t grid = P.wideChunks;
     const uint32_t clearGrid = (uint32_t)std::min<uint64_t>(
-        (ca.clear_words / 4 + 256 * 16 - 1) / (256 * 16), (uint64_t)e->numCUs * 8);
+        (ca.clear_words / 4 + 1024 * 16 - 1) / (1024 * 16), (uint64_t)e->numCUs * 2);
     grid = std::max(grid, clearGrid);
-    lane.multiDirty = true;
-    sortStatus_t st = mgpu_launch(e, lane.stream, e->fnMultiCount, grid, 256, &ca);
+    if (!P.ownMulti) lane.multiDirty = true;
+    sortStatus_t st = mgpu_launch(e, lane.stream, e->fnMultiCountCols, grid, 1024, &ca);
     if (st) return st;
 
     mgpuMultiScanArgs sa;
@@ -405,7 +499,7 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
     sa.pad_ = 0;
     st = mgpu_launch(e, lane.stream, e->fnMultiScan, numWide, 256, &sa);
     if (st) return st;
-    lane.multiDirty = false;
+    if (!P.ownMulti) lane.multiDirty = false;
 
     const int carried = P.valueCount == -1 ? 1 : P.valueCount;
     for (uint32_t p = 0; p < np; ++p) {
@@ -415,9 +509,10 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         pa.tile_begin = P.devTileBegin;
         pa.tile_seg = P.devTileSeg;
         pa.digit_start = digitStart;
-        pa.lookback = lane.lookback + (uint64_t)p * P.wideTiles * MGPU_RADIX;
+        pa.lookback = lookback + (uint64_t)p * P.wideTiles * MGPU_RADIX;
         pa.tile_counter = lane.tileCounters + MGPU_TILE_COUNTER(p, 0);
-        pa.error_flag = e->errorFlag;
+        pa.error_flag = lane.errorDev;
+        pa.spin_limit = e->spinLimit;
         pa.num_segs = numWide;
         pa.num_tiles = P.wideTiles;
         pa.pass = p;
@@ -430,8 +525,19 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         else if (P.valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
         else if (P.valueCount == 1) mode = MGPU_VALUES_SINGLE;
         else mode = MGPU_VALUES_MULTI;
-        st = mgpu_launch(e, lane.stream, P.wideVariant->multi[mode], P.wideTiles,
-                         P.wideVariant->blockThreads, &pa);
+        if (P.wideXcds > 1) {
+            // Workgroups that loop over their XCD's tiles: as many as fit.
+            pa.xcd_tile_begin = P.devXcdTileBegin;
+            pa.num_xcds = P.wideXcds;
+            pa.finished = lane.tileCounters + MGPU_FINISHED_SLOT + p;
+            const uint32_t g =
+                std::max(P.wideXcds, std::min(P.wideTiles, P.wideVariant->multiXGrid));
+            st = mgpu_launch(e, lane.stream, P.wideVariant->multiX[mode], g,
+                             P.wideVariant->blockThreads, &pa);
+        } else {
+            st = mgpu_launch(e, lane.stream, P.wideVariant->multi[mode], P.wideTiles,
+                             P.wideVariant->blockThreads, &pa);
+        }
         if (st) return st;
     }
     return SORT_STATUS_SUCCESS;
@@ -462,7 +568,7 @@ sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
         a.big_list = P.devLists + L.bigOffset;
         a.small_list = P.devLists + L.smallOffset;
         a.counters = P.devCounters + 4 * li;
-        a.error_flag = e->errorFlag;
+        a.error_flag = lane.errorDev;
         a.num_big = L.numBig;
         a.num_small = L.numSmall;
         a.num_values = P.valueCount == -1 ? 1u : (uint32_t)P.valueCount;
@@ -475,7 +581,7 @@ sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
             a.mask[p] = (1u << L.widths[p]) - 1u;
             bit += L.widths[p];
         }
-        sortStatus_t st = mgpu_launch(e, ls, L.fn, L.grid, 256, &a);
+        sortStatus_t st = mgpu_launch(e, ls, L.fn, L.grid, L.block, &a);
         if (st) return st;
     }
     if (!P.wide.empty()) {
@@ -498,9 +604,12 @@ void mgpu_seg_release(SegPlanData& P) {
     if (P.graphExec) (void)hipGraphExecDestroy(P.graphExec);
     if (P.graph) (void)hipGraphDestroy(P.graph);
     if (P.devBlock) (void)hipFree(P.devBlock);
+    if (P.ownLookback) (void)hipFree(P.ownLookback);
+    if (P.ownMulti) (void)hipFree(P.ownMulti);
     P.graphExec = nullptr;
     P.graph = nullptr;
     P.devBlock = nullptr;
+    P.ownLookback = P.ownMulti = nullptr;
 }
 
 // ---------------------------------------------------------------------------
@@ -520,10 +629,12 @@ sortStatus_t validate_segments(const sortData_d* d, int numSegments, const int* 
         if (starts[numSegments] > d->numElements) return SORT_STATUS_INVALID_VALUE;
     }
     if (!d->keys[0] || !d->keys[1]) return SORT_STATUS_INVALID_VALUE;
+    // Both sides of every carried array, in index mode too: a segment's
+    // passes may end on side 0, and a workgroup that runs more passes than
+    // sortArray would ping-pongs through it.
     const int carried = d->valueCount == -1 ? 1 : d->valueCount;
     for (int v = 0; v < carried; ++v)
-        if (!d->values[v][1] || (d->valueCount != -1 && !d->values[v][0]))
-            return SORT_STATUS_INVALID_VALUE;
+        if (!d->values[v][1] || !d->values[v][0]) return SORT_STATUS_INVALID_VALUE;
     return SORT_STATUS_SUCCESS;
 }
 
@@ -580,7 +691,7 @@ sortStatus_t build_orientation(sortSegPlan_d* p, int o) {
     sortEngine_d* e = p->engine;
     SegPlanData& P = p->data[o];
     sortStatus_t st = mgpu_seg_build(e, e->lane, segs.data(), p->numSegments, d.valueCount,
-                                     d.firstBit, d.endBit, &P);
+                                     d.firstBit, d.endBit, &P, nullptr, /*ownScratch=*/true);
     if (st) return st;
     SortLane& lane = e->lane;
     if (lane.histDirty) {  // no memset nodes inside the graph
@@ -630,6 +741,7 @@ sortStatus_t SORTAPI sortSegmentedArray(sortEngine_t engine, sortData_t data, in
     sortStatus_t st = validate_segments(data, numSegments, segmentStarts);
     if (st) return st;
     if (data->endBit == data->firstBit || numSegments == 0) return SORT_STATUS_SUCCESS;
+    if (mgpu_lane_take_error(engine->lane)) return SORT_STATUS_KERNEL_ERROR;
     std::vector<mgpuSegDesc> segs = descs_from_data(data, numSegments, segmentStarts);
     SegPlanData P;
     st = mgpu_seg_build(engine, engine->lane, segs.data(), numSegments, data->valueCount,
@@ -638,7 +750,10 @@ sortStatus_t SORTAPI sortSegmentedArray(sortEngine_t engine, sortData_t data, in
     if (!st && hipStreamSynchronize(engine->lane.stream) != hipSuccess)
         st = SORT_STATUS_DEVICE_ERROR;
     mgpu_seg_release(P);
+    // The buffers were sorted (or not) either way: swap them as a finished
+    // sort does, and report a failed self-check.
     if (!st) finish_swap(data, P.resultSide);
+    if (!st) st = mgpu_lane_take_error(engine->lane);
     return st;
 }
 
@@ -660,13 +775,13 @@ sortStatus_t SORTAPI sortSegmentedBatch(sortEngine_t engine, const sortSegment_d
         for (int side = 0; side < 2; ++side) {
             s.keys[side] = reinterpret_cast<uint32_t*>(in.keys[side]);
             for (int v = 0; v < carried; ++v) {
-                if (in.numElements && !in.values[v][side] && !(valueCount == -1 && side == 0))
-                    return SORT_STATUS_INVALID_VALUE;
+                if (in.numElements && !in.values[v][side]) return SORT_STATUS_INVALID_VALUE;
                 s.vals[v][side] = reinterpret_cast<uint32_t*>(in.values[v][side]);
             }
         }
         s.n = (uint32_t)in.numElements;
     }
+    if (mgpu_lane_take_error(engine->lane)) return SORT_STATUS_KERNEL_ERROR;
     SegPlanData P;
     sortStatus_t st = mgpu_seg_build(engine, engine->lane, segs.data(), numSegments, valueCount,
                                      firstBit, endBit, &P);
@@ -674,6 +789,7 @@ sortStatus_t SORTAPI sortSegmentedBatch(sortEngine_t engine, const sortSegment_d
     if (!st && hipStreamSynchronize(engine->lane.stream) != hipSuccess)
         st = SORT_STATUS_DEVICE_ERROR;
     mgpu_seg_release(P);
+    if (!st) st = mgpu_lane_take_error(engine->lane);
     return st;
 }
 
@@ -703,6 +819,9 @@ sortStatus_t SORTAPI sortSegmentedPlanExecute(sortSegPlan_t plan, sortData_t dat
     if (!plan || !data) return SORT_STATUS_INVALID_VALUE;
     const sortData_d& L = plan->layout;
     if (L.endBit == L.firstBit || plan->numSegments == 0) return SORT_STATUS_SUCCESS;
+    // As sortArray: an unreported failure of earlier work, before anything
+    // is queued.
+    if (mgpu_lane_take_error(plan->engine->lane)) return SORT_STATUS_KERNEL_ERROR;
     // The data must hold the plan's buffers, in either orientation.
     int o;
     if (data->keys[0] == L.keys[0] && data->keys[1] == L.keys[1]) o = 0;

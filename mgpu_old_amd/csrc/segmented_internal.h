@@ -13,14 +13,25 @@ struct SortLane;
 struct PassVariant;
 
 // Kernel variants of mgpusort_seg.hsaco: radix bits 4..8 x items per thread
-// {8, 16}, named mgpu_segsort_r<bits>_i<items>.
+// {8, 16} x workgroup size {128 (up to 7 bits), 256, 512}, named
+// mgpu_segsort_r<bits>_i<items>_b<threads>.
 constexpr int kSegMinBits = 4;
 constexpr int kSegMaxBits = 8;
-constexpr int kSegVariants = (kSegMaxBits - kSegMinBits + 1) * 2;
+constexpr int kSegBlocks = 3;
+constexpr int kSegVariants = (kSegMaxBits - kSegMinBits + 1) * 2 * kSegBlocks;
 
-inline int seg_variant_index(int rb, int ipt) { return (rb - kSegMinBits) * 2 + (ipt == 16); }
-inline int seg_variant_bits(int v) { return kSegMinBits + v / 2; }
-inline int seg_variant_ipt(int v) { return (v & 1) ? 16 : 8; }
+inline int seg_block_index(int block) { return block == 128 ? 0 : block == 512 ? 2 : 1; }
+inline int seg_block_threads(int b) { return b == 0 ? 128 : b == 2 ? 512 : 256; }
+inline bool seg_variant_valid(int rb, int ipt, int block) {
+    return rb >= kSegMinBits && rb <= kSegMaxBits && (ipt == 8 || ipt == 16) &&
+           (block == 128 || block == 256 || block == 512) && block >= (1 << rb);
+}
+inline int seg_variant_index(int rb, int ipt, int block) {
+    return ((rb - kSegMinBits) * 2 + (ipt == 16)) * kSegBlocks + seg_block_index(block);
+}
+inline int seg_variant_bits(int v) { return kSegMinBits + v / (2 * kSegBlocks); }
+inline int seg_variant_ipt(int v) { return ((v / kSegBlocks) & 1) ? 16 : 8; }
+inline int seg_variant_block(int v) { return seg_block_threads(v % kSegBlocks); }
 
 // One row of the tuning table: segments with ceil(log2 n) <= maxLog2 use
 // this variant (rows sorted by maxLog2).
@@ -28,6 +39,16 @@ struct SegTuneEntry {
     int maxLog2;
     int radixBits;
     int itemsPerThread;
+    int blockThreads;
+};
+
+// The device-wide passes of a segmented sort: segments of at least minKeys
+// keys take them; digit_count reads pieces of at most chunk keys; tile shape
+// (0: small unless segments average four large tiles, 1 small, 2 large).
+struct SegWideTune {
+    uint32_t minKeys = 4096 + 1;
+    uint32_t chunk = 1u << 16;
+    int tile = 0;
 };
 
 // One persistent launch of one kernel variant.
@@ -35,6 +56,7 @@ struct SegLaunch {
     int variant = 0;
     hipFunction_t fn = nullptr;
     uint32_t grid = 0;
+    uint32_t block = 256;
     uint32_t bigOffset = 0, smallOffset = 0;
     uint32_t numBig = 0, numSmall = 0;
     int passes = 0;
@@ -71,12 +93,23 @@ struct SegPlanData {
     uint32_t* devTileBegin = nullptr;
     uint32_t* devTileSeg = nullptr;  // [wideTiles]: the device-wide segment of each tile
     uint32_t* devChunks = nullptr;
+    // Lookback chains per XCD (wideXcds > 1): the device-wide segments are
+    // grouped by XCD, XCD x holding global tiles [xcdTileBegin[x],
+    // xcdTileBegin[x + 1]).
+    uint32_t wideXcds = 1;
+    std::vector<uint32_t> xcdTileBegin;
+    uint32_t* devXcdTileBegin = nullptr;
     void* devBlock = nullptr;  // owned allocation (null when an arena holds the arrays)
     mgpuSegDesc* devSegs = nullptr;
     uint32_t* devLists = nullptr;
     uint32_t* devCounters = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graphExec = nullptr;
+    // Scratch of the device-wide passes: the lane's (grow-only, so a later
+    // call on the lane may free and replace it) unless the plan owns its
+    // own, which a captured plan does: its graph's kernels address it.
+    uint32_t* ownLookback = nullptr;  // [passes][wideTiles][MGPU_RADIX]
+    uint32_t* ownMulti = nullptr;     // hist then digit starts, [wide][4][MGPU_RADIX] each
 };
 
 sortStatus_t mgpu_seg_load(sortEngine_d* e);
@@ -85,9 +118,12 @@ void mgpu_seg_unload(sortEngine_d* e);
 // allocation of their own (synchronous copies); with one they are copied
 // asynchronously on lane.stream into the arena, which must not be in use by
 // earlier work still running on another stream.
+// ownScratch: the plan allocates the scratch of its device-wide passes
+// instead of using the lane's (plans that are captured and replayed).
 sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* segs,
                             int numSegments, int valueCount, int firstBit, int endBit,
-                            SegPlanData* out, SegArena* arena = nullptr);
+                            SegPlanData* out, SegArena* arena = nullptr,
+                            bool ownScratch = false);
 void mgpu_arena_free(SegArena* a);
 // Grows an arena (synchronizing `stream` first, whose work may still read the
 // old one) to hold `payload` bytes of plan arrays and `counterWords` queue

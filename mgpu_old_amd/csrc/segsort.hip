@@ -9,7 +9,7 @@
 //
 // Work is split by size on the host:
 //   small  (n <= 64*IPT)     one wave sorts the segment in LDS, all passes;
-//   block  (n <= 256*IPT)    one workgroup sorts it in LDS, all passes;
+//   block  (n <= BLOCK*IPT)  one workgroup sorts it in LDS, all passes;
 //   global (larger)          one workgroup runs every pass through global
 //                            memory, tile by tile, with running digit offsets
 //                            (no cross-workgroup lookback needed).
@@ -34,8 +34,6 @@ using mgpu::st_agent;
 using mgpu::wave_inclusive_scan;
 using mgpu::wave_rank;
 
-constexpr int BLOCK = 256;
-constexpr int W = BLOCK / 64;
 
 // Orders this wave's LDS accesses before and after the call.
 __device__ __forceinline__ void wave_sync() {
@@ -44,8 +42,10 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <int RB, int IPT>
+// Shared memory of one workgroup of BLOCK threads (W waves).
+template <int RB, int IPT, int BLOCK>
 struct SegShared {
+    static constexpr int W = BLOCK / 64;
     static constexpr int RADIX = 1 << RB;
     static constexpr int TILE = BLOCK * IPT;
     static constexpr int MAXP = (32 + RB - 1) / RB;
@@ -94,10 +94,10 @@ __device__ __forceinline__ void emit(const mgpuSegArgs& a, const mgpuSegDesc& s,
 }
 
 // ---- one wave, whole segment in registers + this wave's LDS slice ---------
-template <int RB, int IPT>
-__device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT>& sh,
+template <int RB, int IPT, int BLOCK>
+__device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT, BLOCK>& sh,
                           uint32_t lane, uint32_t wave) {
-    constexpr int RADIX = SegShared<RB, IPT>::RADIX;
+    constexpr int RADIX = SegShared<RB, IPT, BLOCK>::RADIX;
     constexpr int DPL = RADIX >= 64 ? RADIX / 64 : 1;  // digits per lane in the scan
     const uint32_t n = s.n;
     uint32_t* stage = sh.stage + wave * 64 * IPT;
@@ -165,10 +165,11 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
 }
 
 // ---- one workgroup, whole segment in registers + LDS ----------------------
-template <int RB, int IPT>
-__device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT>& sh,
+template <int RB, int IPT, int BLOCK>
+__device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT, BLOCK>& sh,
                            uint32_t tid, uint32_t lane, uint32_t wave) {
-    constexpr int RADIX = SegShared<RB, IPT>::RADIX;
+    constexpr int RADIX = SegShared<RB, IPT, BLOCK>::RADIX;
+    constexpr int W = BLOCK / 64;
     const uint32_t n = s.n;
     uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
     const uint32_t* ki = s.keys[0];
@@ -227,11 +228,12 @@ __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared
 }
 
 // ---- one workgroup, passes through global memory ---------------------------
-template <int RB, int IPT>
-__device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT>& sh,
+template <int RB, int IPT, int BLOCK>
+__device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT, BLOCK>& sh,
                             uint32_t tid, uint32_t lane, uint32_t wave) {
-    constexpr int RADIX = SegShared<RB, IPT>::RADIX;
-    constexpr uint32_t TILE = SegShared<RB, IPT>::TILE;
+    constexpr int RADIX = SegShared<RB, IPT, BLOCK>::RADIX;
+    constexpr int W = BLOCK / 64;
+    constexpr uint32_t TILE = SegShared<RB, IPT, BLOCK>::TILE;
     const uint32_t n = s.n;
     const uint32_t np = a.num_passes;
 
@@ -308,7 +310,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
                     const uint32_t k = sh.stage[q];
                     dest[j] = sh.dst[(k >> shift) & mask] + q;
                     if (dest[j] < n) ko[dest[j]] = k;
-                    else st_agent(a.error_flag, 4u);
+                    else mgpu::set_error(a.error_flag, 4u);
                 }
             }
             for (uint32_t v = 0; v < nv; ++v) {
@@ -352,11 +354,13 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
     }
 }
 
-template <int RB, int IPT>
+template <int RB, int IPT, int BLOCK>
 __device__ void segsort_body(const mgpuSegArgs& a) {
-    __shared__ SegShared<RB, IPT> sh;
+    static_assert(BLOCK >= (1 << RB), "a thread per digit in the block scans");
+    constexpr int W = BLOCK / 64;
+    __shared__ SegShared<RB, IPT, BLOCK> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (uint32_t i = tid; i < (uint32_t)(W * SegShared<RB, IPT>::RADIX); i += BLOCK)
+    for (uint32_t i = tid; i < (uint32_t)(W * SegShared<RB, IPT, BLOCK>::RADIX); i += BLOCK)
         (&sh.match[0][0])[i] = 0;
 
     // Workgroup phase: block and global segments, largest first.
@@ -369,8 +373,8 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
         __syncthreads();
         if (item >= a.num_big) break;
         const mgpuSegDesc& s = a.segs[a.big_list[item]];
-        if (s.n <= (uint32_t)SegShared<RB, IPT>::TILE) block_sort<RB, IPT>(a, s, sh, tid, lane, wave);
-        else global_sort<RB, IPT>(a, s, sh, tid, lane, wave);
+        if (s.n <= (uint32_t)SegShared<RB, IPT, BLOCK>::TILE) block_sort<RB, IPT, BLOCK>(a, s, sh, tid, lane, wave);
+        else global_sort<RB, IPT, BLOCK>(a, s, sh, tid, lane, wave);
         __syncthreads();
     }
 
@@ -382,7 +386,7 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
                                           __HIP_MEMORY_SCOPE_AGENT);
         item = __shfl(item, 0, 64);
         if (item >= a.num_small) break;
-        wave_sort<RB, IPT>(a, a.segs[a.small_list[item]], sh, lane, wave);
+        wave_sort<RB, IPT, BLOCK>(a, a.segs[a.small_list[item]], sh, lane, wave);
     }
 
     __syncthreads();
@@ -400,18 +404,32 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
 
 }  // namespace
 
-#define MGPU_SEGSORT(RB, IPT)                                                                  \
-    extern "C" __global__ void __launch_bounds__(256) mgpu_segsort_r##RB##_i##IPT(mgpuSegArgs a) { \
-        segsort_body<RB, IPT>(a);                                                              \
+// Kernel variants: digit width RB (4..8 bits) x keys per thread IPT (8, 16)
+// x workgroup size BLOCK (128 for digits up to 7 bits, 256, 512).
+#define MGPU_SEGSORT(RB, IPT, B)                                                                \
+    extern "C" __global__ void __launch_bounds__(B)                                            \
+        mgpu_segsort_r##RB##_i##IPT##_b##B(mgpuSegArgs a) {                                    \
+        segsort_body<RB, IPT, B>(a);                                                           \
     }
+#define MGPU_SEGSORT_BLOCKS(RB, IPT)                                                           \
+    MGPU_SEGSORT(RB, IPT, 256)                                                                 \
+    MGPU_SEGSORT(RB, IPT, 512)
 
-MGPU_SEGSORT(4, 8)
-MGPU_SEGSORT(4, 16)
-MGPU_SEGSORT(5, 8)
-MGPU_SEGSORT(5, 16)
-MGPU_SEGSORT(6, 8)
-MGPU_SEGSORT(6, 16)
-MGPU_SEGSORT(7, 8)
-MGPU_SEGSORT(7, 16)
-MGPU_SEGSORT(8, 8)
-MGPU_SEGSORT(8, 16)
+MGPU_SEGSORT(4, 8, 128)
+MGPU_SEGSORT(4, 16, 128)
+MGPU_SEGSORT(5, 8, 128)
+MGPU_SEGSORT(5, 16, 128)
+MGPU_SEGSORT(6, 8, 128)
+MGPU_SEGSORT(6, 16, 128)
+MGPU_SEGSORT(7, 8, 128)
+MGPU_SEGSORT(7, 16, 128)
+MGPU_SEGSORT_BLOCKS(4, 8)
+MGPU_SEGSORT_BLOCKS(4, 16)
+MGPU_SEGSORT_BLOCKS(5, 8)
+MGPU_SEGSORT_BLOCKS(5, 16)
+MGPU_SEGSORT_BLOCKS(6, 8)
+MGPU_SEGSORT_BLOCKS(6, 16)
+MGPU_SEGSORT_BLOCKS(7, 8)
+MGPU_SEGSORT_BLOCKS(7, 16)
+MGPU_SEGSORT_BLOCKS(8, 8)
+MGPU_SEGSORT_BLOCKS(8, 16)

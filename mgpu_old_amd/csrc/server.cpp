@@ -215,7 +215,12 @@ sortStatus_t launch_batch(sortServer_d* s, Slot& slot, std::vector<Pending>& req
     return st;
 }
 
+// Completes every request of the slot's batch with status st, or with
+// SORT_STATUS_KERNEL_ERROR when a kernel of the batch failed a self-check
+// (the slot's own error word, cleared for its next batch).
 void finish_slot(sortServer_d* s, Slot& slot, sortStatus_t st) {
+    const sortStatus_t kernel = mgpu_lane_take_error(slot.lane);
+    if (!st) st = kernel;
     const double t = s->now_ms();
     std::vector<sortCompletion_d> c;
     c.reserve(slot.reqs.size());

@@ -1,7 +1,14 @@
-"""Autotuner of the segmented sort: picks, per segment-size bucket, the kernel
-variant (digit width 4..8 bits x 8 or 16 keys per thread; block size 256) with
-the shortest replay time, and writes the table the engine loads at creation
-(kernelPath + "mgpusort_seg_tuning.txt").
+"""Autotuner of the segmented sort.  It writes the table the engine loads at
+creation (kernelPath + "mgpusort_seg_tuning.txt"):
+
+  1. per segment-size bucket, the persistent kernel variant with the
+     shortest replay: digit width 4..8 bits x keys per thread (8, 16) x
+     workgroup size (128 threads for digits up to 7 bits, 256, 512), timed
+     with every segment kept in the persistent kernel;
+  2. the device-wide passes, on the batch they serve (4096 segments with
+     sizes log-uniform in [1, 2^16], key + one value): the size from which a
+     segment takes them, the keys per count piece and the tile shape, timed
+     with the bucket rows of step 1 in force.
 
 It plays the role of the reference's offline pass-table generator
 (sort/src/tablegen/tablegen.cpp:9-64 over measured rates in throughputs.txt),
@@ -12,15 +19,21 @@ Usage (on a gfx950 box):  python -m mgpu_old_amd.tune_segsort [--out PATH]
 import argparse
 import json
 import os
+import shutil
 import sys
+import tempfile
 import time
 
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # Buckets by ceil(log2 n): (low, high] in log2, inclusive of high.
-BUCKETS = [(0, 6), (6, 8), (8, 10), (10, 12), (12, 14), (14, 16), (16, 18)]
-VARIANTS = [f"r{rb}i{ipt}" for rb in range(4, 9) for ipt in (8, 16)]
+BUCKETS = [(0, 6), (6, 8), (8, 10), (10, 12), (12, 13)]
+VARIANTS = [f"r{rb}i{ipt}b{b}" for rb in range(4, 9) for ipt in (8, 16) for b in (128, 256, 512)
+            if b >= (1 << rb)]
+WIDE_MIN = [1025, 2049, 4097, 8193]
+WIDE_CHUNK = [1 << 14, 1 << 16]
+WIDE_TILE = ["small", "large"]
 
 
 def bucket_sizes(lo, hi, total, seed):
@@ -39,9 +52,10 @@ def bucket_sizes(lo, hi, total, seed):
     return np.array(sizes, dtype=np.int64)
 
 
-def time_variant(torch, engine, variant, keys, vals, starts, reps):
+def time_plan(torch, engine, keys, vals, starts, reps):
+    """Mean time of `reps` replays of a captured segmented plan (key + one
+    value, bits [0, 32)), after one warm-up replay."""
     import mgpu_old_amd as mg
-    os.environ["MGPUSORT_SEG_VARIANT"] = variant
     n = keys.numel()
     k1, v1 = torch.empty_like(keys), torch.empty_like(vals)
     d = mg.SortData()
@@ -51,7 +65,7 @@ def time_variant(torch, engine, variant, keys, vals, starts, reps):
     d.keys[0], d.keys[1] = keys.data_ptr(), k1.data_ptr()
     d.values[0][0], d.values[0][1] = vals.data_ptr(), v1.data_ptr()
     plan = engine.segmented_plan(d, starts)
-    plan.execute(d)  # warm-up (also builds nothing new: 32-bit keys keep orientation)
+    plan.execute(d)  # warm-up (32-bit keys: 4 passes, the buffers keep their orientation)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(reps):
@@ -63,6 +77,18 @@ def time_variant(torch, engine, variant, keys, vals, starts, reps):
     return dt
 
 
+def engine_on(torch, dirpath, table_text):
+    """An engine over a copy of the code objects with the given table."""
+    import mgpu_old_amd as mg
+    for name in ("mgpusort_radix.hsaco", "mgpusort_seg.hsaco"):
+        shutil.copy(os.path.join(mg.LIB_DIR, name), os.path.join(dirpath, name))
+    with open(os.path.join(dirpath, "mgpusort_seg_tuning.txt"), "w") as f:
+        f.write(table_text)
+    eng = mg.Engine(dirpath + os.sep)
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    return eng
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "lib", "mgpusort_seg_tuning.txt"))
@@ -72,40 +98,68 @@ def main():
 
     import torch
     sys.path.insert(0, os.path.dirname(HERE))
-    import mgpu_old_amd as mg
     from mgpu_old_amd import workloads
 
-    # The table being written must not steer the measurement.
-    if os.path.exists(args.out):
-        os.remove(args.out)
-    engine = mg.Engine()
-    engine.set_stream(torch.cuda.current_stream().cuda_stream)
-    rows, report = [], {}
-    for bi, (lo, hi) in enumerate(BUCKETS):
-        sizes = bucket_sizes(lo, hi, args.elements, seed=500 + bi)
+    tmp = tempfile.mkdtemp(prefix="segtune")
+    report = {"buckets": {}, "wide": {}}
+    try:
+        # 1. Persistent kernel per bucket: every segment stays in it.
+        os.environ["MGPUSORT_SEG_WIDE_MIN"] = str(1 << 30)
+        engine = engine_on(torch, tmp, "# no rows: built-in defaults\n")
+        rows = []
+        for bi, (lo, hi) in enumerate(BUCKETS):
+            sizes = bucket_sizes(lo, hi, args.elements, seed=500 + bi)
+            starts = np.zeros(len(sizes) + 1, dtype=np.int64)
+            starts[1:] = np.cumsum(sizes)
+            n = int(starts[-1])
+            keys = torch.from_numpy(workloads.uniform_keys(n, 7).view(np.int32)).cuda()
+            vals = torch.arange(n, dtype=torch.int32, device="cuda")
+            times = {}
+            for v in VARIANTS:
+                os.environ["MGPUSORT_SEG_VARIANT"] = v
+                times[v] = time_plan(torch, engine, keys, vals, starts, args.reps)
+                print(f"[tune] bucket (2^{lo}, 2^{hi}] {len(sizes)} segs {n} keys {v}: "
+                      f"{times[v] * 1e3:.3f} ms ({n / times[v] / 1e9:.2f} Gkeys/s)", flush=True)
+            os.environ.pop("MGPUSORT_SEG_VARIANT", None)
+            best = min(times, key=times.get)
+            rb, rest = best[1:].split("i")
+            ipt, block = rest.split("b")
+            rows.append((hi, int(rb), int(ipt), int(block)))
+            report["buckets"][f"{lo}-{hi}"] = {"best": best, "segments": len(sizes), "keys": n,
+                                               "ms": {k: t * 1e3 for k, t in times.items()}}
+        engine.close()
+        os.environ.pop("MGPUSORT_SEG_WIDE_MIN", None)
+        bucket_text = "".join(f"{hi} {rb} {ipt} {b}\n" for hi, rb, ipt, b in rows)
+
+        # 2. Device-wide passes on the 4096-segment batch, bucket rows in force.
+        sizes = workloads.log_uniform_sizes(4096, 0, 16, seed=4)
         starts = np.zeros(len(sizes) + 1, dtype=np.int64)
         starts[1:] = np.cumsum(sizes)
         n = int(starts[-1])
-        keys = torch.from_numpy(workloads.uniform_keys(n, 7).view(np.int32)).cuda()
+        keys = workloads.uniform_keys_torch(n, 44, "cuda", torch=torch)
         vals = torch.arange(n, dtype=torch.int32, device="cuda")
-        times = {}
-        for v in VARIANTS:
-            times[v] = time_variant(torch, engine, v, keys, vals, starts, args.reps)
-            print(f"[tune] bucket (2^{lo}, 2^{hi}] {len(sizes)} segs {n} keys {v}: "
-                  f"{times[v] * 1e3:.3f} ms ({n / times[v] / 1e9:.2f} Gkeys/s)", flush=True)
-        best = min(times, key=times.get)
-        rb, ipt = int(best[1]), int(best[3:])
-        rows.append((hi, rb, ipt))
-        report[f"{lo}-{hi}"] = {"best": best, "ms": {k: t * 1e3 for k, t in times.items()},
-                                "segments": len(sizes), "keys": n}
-    os.environ.pop("MGPUSORT_SEG_VARIANT", None)
+        wide = {}
+        for m in WIDE_MIN:
+            for c in WIDE_CHUNK:
+                for t in WIDE_TILE:
+                    line = f"wide {m} {c} {t}"
+                    eng = engine_on(torch, tmp, bucket_text + line + "\n")
+                    wide[line] = time_plan(torch, eng, keys, vals, starts, args.reps)
+                    eng.close()
+                    print(f"[tune] 4096 segments ({n} keys) {line}: {wide[line] * 1e3:.3f} ms",
+                          flush=True)
+        best_wide = min(wide, key=wide.get)
+        report["wide"] = {"best": best_wide, "keys": n, "ms": {k: t * 1e3 for k, t in wide.items()}}
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+
     with open(args.out, "w") as f:
-        f.write("# segmented-sort tuning table: max ceil(log2 n), radix bits, keys per thread\n")
-        f.write("# written by mgpu_old_amd/tune_segsort.py (key + 1 value, bits [0,32))\n")
-        for hi, rb, ipt in rows:
-            f.write(f"{hi} {rb} {ipt}\n")
-    print(json.dumps({"table": rows, "report": report}), flush=True)
-    engine.close()
+        f.write("# segmented-sort tuning table, written by mgpu_old_amd/tune_segsort.py (key + 1 value,\n"
+                "# bits [0,32)): <max ceil(log2 n)> <radix bits> <keys per thread> <threads> per\n"
+                "# size bucket of the persistent kernel; wide <min keys> <count piece> <tile>\n")
+        f.write(bucket_text)
+        f.write(best_wide + "\n")
+    print(json.dumps({"table": rows, "wide": best_wide, "report": report}), flush=True)
 
 
 if __name__ == "__main__":

@@ -105,6 +105,26 @@ def and3_keys_torch(n, seed, device, mask_bits=32, torch=None):
     return out
 
 
+def nearly_sorted_keys(n, seed, shuffled_fraction=0.01, block=16384):
+    """Sorted uniform keys in which about `shuffled_fraction` of the blocks
+    of `block` keys (chosen by the seed) hold their keys in a seeded random
+    order."""
+    keys = np.sort(uniform_keys(n, seed))
+    nb = (n + block - 1) // block
+    pick = words(seed + 1, 0, nb).astype(np.float64) / 4294967296.0 < shuffled_fraction
+    for b in np.flatnonzero(pick):
+        s, e = int(b) * block, min(n, (int(b) + 1) * block)
+        keys[s:e] = keys[s:e][np.argsort(words(seed + 2, s, e - s), kind="stable")]
+    return keys
+
+
+def top_sorted_keys(n, seed, top_bits=8):
+    """Uniform keys in stable order of their top `top_bits` bits only (the
+    bits below stay in random order)."""
+    keys = uniform_keys(n, seed)
+    return keys[np.argsort(keys >> np.uint32(32 - top_bits), kind="stable")]
+
+
 def log_uniform_sizes(count, lo_log2, hi_log2, seed, lo=None, hi=None):
     """count sizes floor(2 ** (lo_log2 + (hi_log2 - lo_log2) * u)), clamped to
     [lo, hi] (default [2**lo_log2, 2**hi_log2])."""

@@ -194,3 +194,29 @@ def test_code_objects_built_for_gfx950():
         assert data[:4] == b"\x7fELF"
         assert b"gfx950" in data
         assert b"mgpu_digit_count" in data and b"mgpu_rank_scatter_b1024_i16_r1_v0" in data
+
+
+@pytest.mark.parametrize("end_bit", [32, 24])
+def test_segmented_index_mode_needs_value_side_zero_without_device(end_bit):
+    """Index generation with a null values[0][0] is rejected by argument
+    validation, before the engine is used (the handle here is a zeroed block
+    standing in for an engine that the validation never reads)."""
+    L = mg.lib()
+    fake = ctypes.create_string_buffer(4096)
+    eng = ctypes.cast(fake, ctypes.c_void_p)
+    d = mg.SortData()
+    d.maxElements = d.numElements = 4096
+    d.valueCount, d.firstBit, d.endBit = -1, 0, end_bit
+    d.keys[0], d.keys[1] = 0x10000, 0x20000
+    d.values[0][1] = 0x30000
+    starts = (ctypes.c_int * 3)(0, 1000, 4096)
+    assert L.sortSegmentedArray(eng, ctypes.byref(d), 2, starts) == mg.SortStatus.INVALID_VALUE
+    plan = ctypes.c_void_p()
+    assert L.sortSegmentedPlanCreate(eng, ctypes.byref(d), 2, starts, ctypes.byref(plan)) == \
+        mg.SortStatus.INVALID_VALUE
+    seg = mg.SortSegment()
+    seg.keys[0], seg.keys[1] = 0x10000, 0x20000
+    seg.values[0][1] = 0x30000
+    seg.numElements = 4096
+    assert L.sortSegmentedBatch(eng, ctypes.byref(seg), 1, -1, 0, end_bit) == \
+        mg.SortStatus.INVALID_VALUE

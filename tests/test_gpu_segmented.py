@@ -159,3 +159,127 @@ def test_batch_of_separate_buffers_alone_vs_mix(engine, oracle):
         assert np.array_equal(mix_k, alone.keys()) and np.array_equal(mix_v, alone.values()[0])
         rk, (rv,) = oracle.stable_sort(k, [v], first_bit=0, end_bit=24)
         assert np.array_equal(mix_k, rk) and np.array_equal(mix_v, rv)
+
+
+def test_plan_keeps_its_scratch_when_the_lane_grows(engine, oracle):
+    """A captured plan addresses scratch of its own: a larger sortArray and a
+    segmented sort with more device-wide arrays on the same engine (both grow
+    the engine's scratch, freeing the old) must not disturb its replay."""
+    import torch
+    sizes = [6000, 50_000, 9000, 120_000]
+    starts = _segments(sizes)
+    n = int(starts[-1])
+    g = GpuSort(np.zeros(n, dtype=np.uint32), [np.zeros(n, dtype=np.uint32)])
+    plan = engine.segmented_plan(g.data, starts)
+    big = GpuSort(workloads.uniform_keys(1 << 26, seed=300))
+    big.run(engine)
+    engine.check_device()
+    many = [5000 + 37 * i for i in range(64)]
+    ms = _segments(many)
+    mk = workloads.uniform_keys(int(ms[-1]), seed=301)
+    _run_segmented(engine, mk, [np.arange(mk.size, dtype=np.uint32)], 1, ms)
+    keys = workloads.uniform_keys(n, seed=302)
+    vals = [workloads.uniform_keys(n, seed=303)]
+    g.by_ptr[g.data.keys[0]][:n].copy_(to_dev(keys)[:n])
+    g.by_ptr[g.data.values[0][0]][:n].copy_(to_dev(vals[0])[:n])
+    torch.cuda.synchronize()
+    plan.execute(g.data)
+    torch.cuda.synchronize()
+    engine.check_device()
+    _check_segments(oracle, keys, vals, 1, starts, g.keys(), g.values(), 0, 32)
+    plan.close()
+    del big
+
+
+@pytest.mark.parametrize("eb", [32, 24, 8])
+def test_index_mode_needs_both_value_sides(engine, eb):
+    """Index generation writes through values[0] on both sides, so a null
+    side 0 is rejected before anything runs (sortArray accepts it only for a
+    one-pass sort; the segmented calls always need it)."""
+    import ctypes
+    g = GpuSort(workloads.uniform_keys(10_000, seed=310), value_count=-1, end_bit=eb)
+    g.data.values[0][0] = 0
+    starts = (ctypes.c_int * 3)(0, 5000, 10_000)
+    L = mg.lib()
+    assert L.sortSegmentedArray(engine.handle, ctypes.byref(g.data), 2, starts) == \
+        mg.SortStatus.INVALID_VALUE
+    seg = mg.SortSegment()
+    seg.keys[0], seg.keys[1] = g.data.keys[0], g.data.keys[1]
+    seg.values[0][1] = g.data.values[0][1]
+    seg.numElements = 10_000
+    assert L.sortSegmentedBatch(engine.handle, ctypes.byref(seg), 1, -1, 0, eb) == \
+        mg.SortStatus.INVALID_VALUE
+    plan = ctypes.c_void_p()
+    assert L.sortSegmentedPlanCreate(engine.handle, ctypes.byref(g.data), 2, starts,
+                                     ctypes.byref(plan)) == mg.SortStatus.INVALID_VALUE
+
+
+@pytest.mark.parametrize("layout", ["xcd", "one_chain", "small_pieces", "dominant_segment"])
+def test_device_wide_layouts(oracle, layout):
+    """The device-wide passes of a segmented sort in each layout: lookback
+    chains per XCD with whole segments grouped by XCD (default), one chain
+    per segment across the device (MGPUSORT_XCD=0, or a segment holding more
+    than a quarter of an XCD's share of the tiles), and count pieces smaller
+    than a segment (MGPUSORT_SEG_WIDE_CHUNK=4096).  Every segment bit-exact
+    against the oracle."""
+    import torch
+    env = {"MGPUSORT_XCD": "0"} if layout == "one_chain" else {}
+    eng = make_engine(env)
+    saved = os.environ.get("MGPUSORT_SEG_WIDE_CHUNK")
+    try:
+        if layout == "small_pieces":
+            os.environ["MGPUSORT_SEG_WIDE_CHUNK"] = "4096"
+        sizes = list(workloads.log_uniform_sizes(300, 12, 17, seed=320))
+        if layout == "dominant_segment":
+            sizes[7] = 1 << 22
+        starts = _segments(sizes)
+        n = int(starts[-1])
+        keys = workloads.uniform_keys(n, seed=321)
+        vals = [workloads.uniform_keys(n, seed=322)]
+        for vc, fb, eb in [(1, 0, 32), (-1, 3, 27)]:
+            vv = vals if vc == 1 else []
+            g = _run_segmented(eng, keys, vv, vc, starts, fb, eb)
+            _check_segments(oracle, keys, vv, vc, starts, g.keys(), g.values(), fb, eb)
+    finally:
+        if saved is None:
+            os.environ.pop("MGPUSORT_SEG_WIDE_CHUNK", None)
+        else:
+            os.environ["MGPUSORT_SEG_WIDE_CHUNK"] = saved
+        eng.close()
+
+
+def test_seg_table_dimensions_honoured(tmp_path, oracle):
+    """Every dimension of a segmented tuning-table row is honoured: digit
+    width, keys per thread and workgroup size per size bucket (read from the
+    names of the kernels that ran), and the wide row's routing threshold and
+    tile shape (its count piece of 1024 keys cuts segments into many pieces);
+    malformed rows and a 128-thread 8-bit variant (not built) are ignored.
+    Every segment bit-exact against the oracle."""
+    import shutil
+    import torch
+    for name in ("mgpusort_radix.hsaco", "mgpusort_seg.hsaco"):
+        shutil.copy(os.path.join(mg.LIB_DIR, name), tmp_path / name)
+    (tmp_path / "mgpusort_seg_tuning.txt").write_text(
+        "# test table\n6 5 8 128\n8 8 8 128\n10 8 16 512\nnonsense\n31 7 16 256\n"
+        "wide 2049 1024 large\n")
+    eng = mg.Engine(str(tmp_path) + "/")
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        sizes = [1, 40, 64, 65, 700, 1024, 1025, 2048, 2049, 5000, 30_000, 3]
+        starts = _segments(sizes)
+        n = int(starts[-1])
+        keys = workloads.uniform_keys(n, seed=330)
+        vals = [workloads.uniform_keys(n, seed=331)]
+        eng.enable_timing(True)
+        eng.read_timing()
+        g = _run_segmented(eng, keys, vals, 1, starts)
+        names = set(eng.read_timing())
+        eng.enable_timing(False)
+        _check_segments(oracle, keys, vals, 1, starts, g.keys(), g.values(), 0, 32)
+        seg_kernels = {k for k in names if k.startswith("mgpu_segsort")}
+        assert seg_kernels == {"mgpu_segsort_r5_i8_b128", "mgpu_segsort_r8_i16_b512",
+                               "mgpu_segsort_r7_i16_b256"}, names
+        scatter = {k for k in names if k.startswith("mgpu_multi_scatter")}
+        assert scatter and all("_b1024_" in k for k in scatter), names
+    finally:
+        eng.close()

@@ -217,3 +217,48 @@ def test_submit_from_several_threads(engine, oracle):
         assert np.array_equal(to_host(tensors[i][0], keys.size), rk), f"request {i} keys"
         for j, t in enumerate(tensors[i][1]):
             assert np.array_equal(to_host(t, keys.size), rv[j]), f"request {i} values[{j}]"
+
+
+def test_served_full_size_law(engine, oracle):
+    """The served workload's whole size law, [2^10, 2^24]: beside requests
+    drawn from it, a dozen in (2^20, 2^24] with one or two value arrays and
+    16-, 24- and 32-bit keys, arriving as a Poisson stream; every request
+    bit-exact against the oracle's sort of that request alone, and every
+    completion SUCCESS."""
+    import torch
+    reqs, host, tensors = _make_requests(24, seed=41, lo_log2=10, hi_log2=24)
+    sizes = workloads.log_uniform_sizes(12, 20, 24, seed=42, lo=(1 << 20) + 1)
+    for i, n in enumerate(sizes):
+        n, w, vc = int(n), (16, 24, 32)[i % 3], 1 + (i % 2)
+        keys = workloads.uniform_keys(n, seed=4300 + i, mask_bits=w)
+        vals = [workloads.uniform_keys(n, seed=4400 + 7 * i + j) for j in range(vc)]
+        tk, tv = to_dev(keys), [to_dev(v) for v in vals]
+        r = mg.SortRequest()
+        r.keys = tk.data_ptr()
+        for j, t in enumerate(tv):
+            r.values[j] = t.data_ptr()
+        r.numElements, r.valueCount, r.firstBit, r.endBit = n, vc, 0, w
+        reqs.append(r)
+        host.append((keys, vals, w))
+        tensors.append((tk, tv))
+    order = np.argsort(workloads.words(44, 0, len(reqs)), kind="stable")
+    reqs = [reqs[i] for i in order]
+    host = [host[i] for i in order]
+    tensors = [tensors[i] for i in order]
+    arrivals = _poisson(len(reqs), 1.0, seed=45)
+    torch.cuda.synchronize()
+    with mg.Server(engine, 1 << 31, max_in_flight=4, max_elements=1 << 24) as srv:
+        comps = srv.replay(reqs, arrivals)
+    torch.cuda.synchronize()
+    big = 0
+    for i, c in enumerate(comps):
+        assert c.status == 0, f"request {i} failed: {mg.status_string(c.status)}"
+        keys, vals, w = host[i]
+        tk, tv = tensors[i]
+        n = keys.size
+        big += n > (1 << 20)
+        rk, rv = oracle.stable_sort(keys, vals, first_bit=0, end_bit=w)
+        assert np.array_equal(to_host(tk, n), rk), f"request {i} (n={n}, {w} bits) keys differ"
+        for j, t in enumerate(tv):
+            assert np.array_equal(to_host(t, n), rv[j]), f"request {i} values[{j}] differ"
+    assert big >= 12

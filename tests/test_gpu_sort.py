@@ -418,3 +418,171 @@ def test_sub_array_pointers(engine, oracle, offset, n):
                     (alt[1], 0x3C3C3C3C)):
         h = t.cpu().numpy().view(np.uint32)
         assert np.all(h[:offset] == fill) and np.all(h[offset + n:] == fill)
+
+
+def test_stream_switch_waits_for_queued_work(oracle):
+    """Two sorts on one engine, the second queued on another stream right
+    after sortSetStream: it must wait for the first (both use the engine's
+    scratch), and both results must be exact."""
+    import torch
+    s1 = torch.cuda.Stream()
+    s2 = torch.cuda.Stream()
+    eng = make_engine()
+    try:
+        k1 = workloads.uniform_keys(1 << 24, seed=400)
+        k2 = workloads.uniform_keys(1 << 24, seed=401)
+        g1, g2 = GpuSort(k1, [np.arange(k1.size, dtype=np.uint32)]), GpuSort(k2)
+        torch.cuda.synchronize()
+        eng.set_stream(s1.cuda_stream)
+        eng.sort_array(g1.data)
+        eng.set_stream(s2.cuda_stream)
+        eng.sort_array(g2.data)
+        torch.cuda.synchronize()
+        eng.check_device()
+        rk, (rv,) = oracle.stable_sort(k1, [np.arange(k1.size, dtype=np.uint32)])
+        assert np.array_equal(g1.keys(), rk) and np.array_equal(g1.values()[0], rv)
+        assert np.array_equal(g2.keys(), np.sort(k2))
+    finally:
+        eng.close()
+
+
+def test_size_beyond_2p29(engine):
+    """n in (2^29, 2^30), where byte offsets of an array pass 2^31: key +
+    generated index, checked on the device by the properties that fix a
+    stable sort (sorted keys, a permutation, keys gathered through it, ties in
+    source order)."""
+    import torch
+    n = (1 << 29) + 12_345
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(29)
+    k0 = torch.randint(-(1 << 31), 1 << 31, (n,), dtype=torch.int32, device="cuda", generator=gen)
+    keys_in = k0.clone()
+    k1, v0, v1 = (torch.empty_like(k0) for _ in range(3))
+    d = mg.SortData()
+    d.maxElements = d.numElements = n
+    d.valueCount, d.firstBit, d.endBit = -1, 0, 32
+    d.keys[0], d.keys[1] = k0.data_ptr(), k1.data_ptr()
+    d.values[0][0], d.values[0][1] = v0.data_ptr(), v1.data_ptr()
+    torch.cuda.synchronize()
+    engine.sort_array(d)
+    torch.cuda.synchronize()
+    engine.check_device()
+    by_ptr = {t.data_ptr(): t for t in (k0, k1, v0, v1)}
+    out_k = by_ptr[d.keys[0]].long() & 0xFFFFFFFF
+    idx = by_ptr[d.values[0][0]].long()
+    assert bool((out_k[1:] >= out_k[:-1]).all()), "keys not sorted"
+    assert int(idx.min()) >= 0 and int(idx.max()) < n
+    seen = torch.zeros(n, dtype=torch.bool, device="cuda")
+    seen[idx] = True
+    assert bool(seen.all()), "indices are not a permutation"
+    assert bool(((keys_in.long() & 0xFFFFFFFF)[idx] == out_k).all()), "keys do not follow indices"
+    ties = out_k[1:] == out_k[:-1]
+    assert bool((idx[1:][ties] > idx[:-1][ties]).all()), "equal keys out of source order"
+
+
+@pytest.fixture(scope="module")
+def ee_engine():
+    """The early-exit kernel for the first pass that runs whenever earlyExit
+    is set (MGPUSORT_EE=1), whatever the descent counts say."""
+    eng = make_engine({"MGPUSORT_EE": "1"})
+    yield eng
+    eng.close()
+
+
+def _ee_kernels_used(engine):
+    return [k for k in engine.read_timing() if "_ee_" in k]
+
+
+@pytest.mark.parametrize("case,fb,eb,executed,ee", [
+    # sorted keys with 0.5% of their 1K-key blocks shuffled: the low digits
+    # are disordered everywhere
+    ("nearly_sorted", 0, 32, 4, False),
+    # ... but bits [18, 26) and [20, 26) have few descents (4-6K over 2K
+    # waves): the first pass takes the early-exit kernel
+    ("nearly_sorted", 18, 26, 1, True),
+    ("nearly_sorted", 20, 32, 2, True),
+    # in order of the top byte only: the low passes scramble that order
+    ("top_sorted", 0, 32, 4, False),
+    # a sort by the top byte of input already in top-byte order: the identity
+    ("top_sorted", 24, 32, 0, False),
+    # by bits [16, 32) of input in order of its top 16 bits but for 64 keys:
+    # pass [16, 24) runs with the early-exit kernel, then pass [24, 32)
+    ("top_sorted_low_run", 16, 32, 2, True),
+    # pass 0's digit is in order in the input (skipped), pass 1's is not
+    ("mid_sorted", 8, 24, 1, False),
+])
+@pytest.mark.parametrize("vc", [-1, 0, 1, 3])
+@pytest.mark.parametrize("which", ["default", "xcd"])
+def test_early_exit_partial_order(request, oracle, case, fb, eb, executed, ee, vc, which):
+    """earlyExit on partially ordered input: a leading pass whose digit is
+    already in order in the input is skipped (the identity); the first pass
+    that runs may take the early-exit kernel, which ranks waves and writes
+    tiles already in digit order without the atomics and the staging.  The
+    output equals the oracle's either way, and parity follows the executed
+    passes."""
+    engine = request.getfixturevalue("engine" if which == "default" else "xcd_engine")
+    n = (1 << 21) + 77
+    if case == "nearly_sorted":
+        keys = workloads.nearly_sorted_keys(n, seed=91, shuffled_fraction=0.005, block=1024)
+    elif case == "top_sorted":
+        keys = workloads.top_sorted_keys(n, seed=92)
+    elif case == "top_sorted_low_run":
+        # top byte in order, bits [16, 24) in order within each top byte for
+        # all but a few keys: pass [16, 24) has few descents
+        keys = workloads.top_sorted_keys(n, seed=93, top_bits=16)
+        k = workloads.words(94, 0, 64) % np.uint32(n)
+        keys[k] = workloads.words(95, 0, 64)
+    else:  # mid_sorted: bits [8, 16) nondecreasing, [16, 24) random
+        keys = workloads.uniform_keys(n, seed=96)
+        mid = np.sort((keys >> np.uint32(8)) & np.uint32(0xFF))
+        keys = (keys & np.uint32(0xFFFF00FF)) | (mid << np.uint32(8))
+    vals = [workloads.uniform_keys(n, seed=97 + j) for j in range(max(vc, 0))]
+    g = GpuSort(keys, vals, value_count=vc, first_bit=fb, end_bit=eb)
+    g.data.earlyExit = True
+    engine.enable_timing(True)
+    engine.read_timing()
+    g.run(engine)
+    used = _ee_kernels_used(engine)
+    engine.enable_timing(False)
+    engine.check_device()
+    _check_vs_oracle(oracle, keys, g, vals, vc, fb, eb)
+    assert g.data.parity == executed % 2
+    assert bool(used) == (ee and executed > 0), used
+
+
+@pytest.mark.parametrize("kind", ["random", "nearly_sorted", "top_sorted", "waves_sorted"])
+@pytest.mark.parametrize("vc", [-1, 0, 1, 2])
+@pytest.mark.parametrize("which", ["ee", "ee_xcd"])
+def test_early_exit_kernel_forced(request, oracle, kind, vc, which):
+    """The early-exit kernel on every kind of input, also where it finds
+    nothing in order (random) and where whole tiles are (waves_sorted: keys
+    whose top byte is in order, sorted by that byte, so every pass-0 tile is
+    ordered).  Bit-exact against the oracle."""
+    env = {"MGPUSORT_EE": "1"}
+    if which == "ee_xcd":
+        env["MGPUSORT_XCD"] = "1"
+    eng = make_engine(env)
+    try:
+        n = 3 * 16384 * 8 + 4097
+        fb, eb = 0, 32
+        if kind == "random":
+            keys = workloads.uniform_keys(n, seed=98)
+        elif kind == "nearly_sorted":
+            keys = workloads.nearly_sorted_keys(n, seed=99, shuffled_fraction=0.3, block=1024)
+            fb, eb = 20, 32
+        elif kind == "top_sorted":
+            keys = workloads.top_sorted_keys(n, seed=100)
+            fb, eb = 16, 32
+        else:
+            keys = np.sort(workloads.uniform_keys(n, seed=101) >> np.uint32(24)) << np.uint32(24)
+            keys |= workloads.uniform_keys(n, seed=102, mask_bits=24)
+            fb, eb = 24, 32
+            keys[5] ^= np.uint32(0xFF000000)  # one descent: the pass runs
+        vals = [workloads.uniform_keys(n, seed=103 + j) for j in range(max(vc, 0))]
+        g = GpuSort(keys, vals, value_count=vc, first_bit=fb, end_bit=eb)
+        g.data.earlyExit = True
+        g.run(eng)
+        eng.check_device()
+        _check_vs_oracle(oracle, keys, g, vals, vc, fb, eb)
+    finally:
+        eng.close()
