@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-segmented", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-early-exit", action="store_true")
+    ap.add_argument("--no-headline", action="store_true",
+                    help="skip the headline sort (profiling the other legs alone); value is null")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-log2n", type=int, default=25)
     ap.add_argument("--served-requests", type=int, default=600)
@@ -471,32 +473,36 @@ def main():
         log(f"[bench] copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
 
     n = 1 << args.log2n
-    keys = workloads.uniform_keys_torch(n, 2028, "cuda", torch=torch)
-    vals = torch.arange(n, dtype=torch.int32, device="cuda")
-    head = run_workload(torch, eng, f"sortArray 2^{args.log2n} u32 key + 1 u32 value (0..N-1), "
-                        "uniform keys, bits [0,32)", keys, [vals], 1, 0, 32, args.steps,
-                        args.warmup)
-    timed = run_workload(torch, eng, "per-kernel timing", keys, [vals], 1, 0, 32,
-                         min(args.steps, 5), 1, timing=True, verify=False)
-    del keys, vals
-    kt = timed["kernels"]
-    scatter = {k: v for k, v in kt.items() if k.startswith("mgpu_rank_scatter")}
-    name, (launches, total_ms, mn, mx) = max(scatter.items(), key=lambda kv: kv[1][1])
-    avg_s = total_ms / launches / 1e3
-    per_launch = 8 * n * 2  # read + write of the keys and one value array
-    roof = {"bound": "hbm", "kernel": name, "achieved": per_launch / avg_s / 1e9,
-            "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per_launch / avg_s / HBM_PEAK,
-            "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": launches,
-            "bytes_per_launch": per_launch}
-    try:
-        with open(TRAFFIC_FILE) as f:
-            measured = json.load(f)
-        if measured.get("kernel") == name and measured.get("n") == n:
-            roof["traffic"] = measured["traffic_bytes"]
-            roof["traffic_source"] = measured["source"]
-    except (OSError, ValueError, KeyError):
-        pass
-    kernel_ms = {k: {"avg_ms": v[1] / v[0], "launches": v[0]} for k, v in kt.items()}
+    head = {"gkeys_per_s": None, "ms_per_sort": None, "workload": None, "passes": None,
+            "bit_exact_properties": None, "sort_roofline": None}
+    roof, kernel_ms = None, {}
+    if not args.no_headline:
+        keys = workloads.uniform_keys_torch(n, 2028, "cuda", torch=torch)
+        vals = torch.arange(n, dtype=torch.int32, device="cuda")
+        head = run_workload(torch, eng, f"sortArray 2^{args.log2n} u32 key + 1 u32 value (0..N-1), "
+                            "uniform keys, bits [0,32)", keys, [vals], 1, 0, 32, args.steps,
+                            args.warmup)
+        timed = run_workload(torch, eng, "per-kernel timing", keys, [vals], 1, 0, 32,
+                             min(args.steps, 5), 1, timing=True, verify=False)
+        del keys, vals
+        kt = timed["kernels"]
+        scatter = {k: v for k, v in kt.items() if k.startswith("mgpu_rank_scatter")}
+        name, (launches, total_ms, mn, mx) = max(scatter.items(), key=lambda kv: kv[1][1])
+        avg_s = total_ms / launches / 1e3
+        per_launch = 8 * n * 2  # read + write of the keys and one value array
+        roof = {"bound": "hbm", "kernel": name, "achieved": per_launch / avg_s / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": per_launch / avg_s / HBM_PEAK,
+                "traffic": None, "avg_launch_ms": avg_s * 1e3, "launches": launches,
+                "bytes_per_launch": per_launch}
+        try:
+            with open(TRAFFIC_FILE) as f:
+                measured = json.load(f)
+            if measured.get("kernel") == name and measured.get("n") == n:
+                roof["traffic"] = measured["traffic_bytes"]
+                roof["traffic_source"] = measured["source"]
+        except (OSError, ValueError, KeyError):
+            pass
+        kernel_ms = {k: {"avg_ms": v[1] / v[0], "launches": v[0]} for k, v in kt.items()}
 
     extra = []
     if not args.no_extra:

@@ -173,16 +173,19 @@ sortStatus_t mgpu_lane_init(SortLane* l) {
     // The error word: coherent pinned memory, so a kernel's store reaches the
     // host without a copy and nothing on the normal path pays for it.
     void* word = nullptr;
-    if (hipHostMalloc(&word, 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+    if (hipHostMalloc(&word, kLanePinnedBytes, hipHostMallocMapped | hipHostMallocCoherent) !=
+        hipSuccess) {
         l->errorWord = nullptr;
         return SORT_STATUS_HOST_ALLOC_FAILED;
     }
     l->errorWord = static_cast<uint32_t*>(word);
-    std::memset(word, 0, 64);
+    std::memset(word, 0, kLanePinnedBytes);
     void* dev = nullptr;
     if (hipHostGetDevicePointer(&dev, word, 0) != hipSuccess || !dev)
         return SORT_STATUS_DEVICE_ERROR;
     l->errorDev = static_cast<uint32_t*>(dev);
+    l->reportWord = l->errorWord + kLaneReportOffset;
+    l->reportDev = l->errorDev + kLaneReportOffset;
     const size_t words = 2 * kLaneHistWords + MGPU_LANE_COUNTERS + kLaneSegCountWords;
     if (hipMalloc(&l->fixed, words * sizeof(uint32_t)) != hipSuccess) {
         l->fixed = nullptr;
@@ -219,7 +222,7 @@ void mgpu_lane_free(SortLane* l) {
     l->lookbackWords = 0;
     l->fixed = nullptr;
     if (l->errorWord) (void)hipHostFree(l->errorWord);
-    l->errorWord = l->errorDev = nullptr;
+    l->errorWord = l->errorDev = l->reportWord = l->reportDev = nullptr;
 }
 
 uint32_t mgpu_lane_peek_error(const SortLane& l) {
@@ -387,8 +390,24 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
                      timed);
     if (st) return st;
 
+    // digit_scan (also puts hist and the counters back to zero; with early
+    // exit it also fills the lane's host-visible report).
+    mgpuScanArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.hist = L.hist;
+    sa.digit_start = L.digitStart;
+    sa.tile_counters = L.tileCounters;
+    sa.num_passes = (uint32_t)numPasses;
+    sa.num_segs = segs;
+    sa.seg_count = L.segCount;
+    sa.seg_count_words = segs > 1 ? (uint32_t)kLaneSegCountWords : 0u;
+    sa.report = earlyExit ? L.reportDev : nullptr;
+    st = mgpu_launch(e, L.stream, e->fnScan, (unsigned)numPasses, 256, &sa, timed);
+    if (st) return st;
+
     // Early exit (the role of the reference's sortDetectCounters readback and
-    // its codes 1-3, mgpusort.cpp:433-451, for all passes at once):
+    // its codes 1-3, mgpusort.cpp:433-451, for all passes at once), from the
+    // scan's report, read once the stream is idle:
     //   * already-sorted input (by the whole bit field) skips every pass;
     //   * a pass whose digit is the same for every key is the identity;
     //   * so is the first pass when the input is already in its digit's
@@ -401,28 +420,15 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     //     sorted input) leaves more.
     bool run[MGPU_MAX_PASSES];
     for (int p = 0; p < numPasses; ++p) run[p] = true;
-    bool sorted = false;
     int eePass = -1;
     if (earlyExit) {
-        const size_t hw = (size_t)segs * MGPU_MAX_PASSES * MGPU_RADIX;
-        std::vector<uint32_t> h(hw + MGPU_LANE_COUNTERS);
-        if (hipMemcpyAsync(h.data(), L.hist, hw * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           L.stream) != hipSuccess ||
-            hipMemcpyAsync(h.data() + hw, L.tileCounters, MGPU_LANE_COUNTERS * sizeof(uint32_t),
-                           hipMemcpyDeviceToHost, L.stream) != hipSuccess ||
-            hipStreamSynchronize(L.stream) != hipSuccess)
-            return SORT_STATUS_DEVICE_ERROR;
-        sorted = h[hw + MGPU_UNSORTED_SLOT] == 0;
+        if (hipStreamSynchronize(L.stream) != hipSuccess) return SORT_STATUS_DEVICE_ERROR;
+        const volatile uint32_t* rep = L.reportWord;
+        const bool sorted = rep[0] == 0;
+        const uint32_t firstDescents = rep[1];
         bool leading = true;  // no pass has run yet: the input is as given
-        const uint32_t firstDescents = h[hw + MGPU_UNSORTED_SLOT + 1];
         for (int p = 0; p < numPasses; ++p) {
-            uint32_t most = 0;
-            for (uint32_t d = 0; d < MGPU_RADIX; ++d) {
-                uint32_t c = 0;
-                for (uint32_t x = 0; x < segs; ++x)
-                    c += h[(x * MGPU_MAX_PASSES + p) * MGPU_RADIX + d];
-                most = std::max(most, c);
-            }
+            const uint32_t most = rep[2 + p];
             // Descents are counted for pass 0's digit only: what a later
             // leading pass would find in the input is not known.
             const bool inOrder = p == 0 && firstDescents == 0;
@@ -435,19 +441,6 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
             }
         }
     }
-
-    // digit_scan (also puts hist and the counters back to zero)
-    mgpuScanArgs sa;
-    sa.hist = L.hist;
-    sa.digit_start = L.digitStart;
-    sa.tile_counters = L.tileCounters;
-    sa.num_passes = (uint32_t)numPasses;
-    sa.num_segs = segs;
-    sa.seg_count = L.segCount;
-    sa.seg_count_words = segs > 1 ? (uint32_t)kLaneSegCountWords : 0u;
-    sa.pad_ = 0;
-    st = mgpu_launch(e, L.stream, e->fnScan, (unsigned)numPasses, 256, &sa, timed);
-    if (st) return st;
     L.histDirty = false;
 
     // rank_scatter per executed pass; executed pass k reads side k & 1.

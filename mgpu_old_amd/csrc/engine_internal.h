@@ -68,12 +68,21 @@ struct SortLane {
     // its XCD.  The host reads it without a copy.
     uint32_t* errorWord = nullptr;  // host address
     uint32_t* errorDev = nullptr;   // the same word as the kernels address it
+    // The early-exit report digit_scan writes in the same pinned block (its
+    // own cache line): descents and the largest digit count of each pass.
+    uint32_t* reportWord = nullptr;
+    uint32_t* reportDev = nullptr;
     hipEvent_t streamSwitch = nullptr;  // orders a new stream after the old (sortSetStream)
     // A view over caller-provided scratch (sortArrayNoAlloc): the passes
     // never allocate, clear with a memset, read back or time anything; a sort
     // whose lookback words do not fit lookbackWords is refused.
     bool noAlloc = false;
 };
+
+// The lane's pinned, device-mapped block: the error word at 0, the early-exit
+// report from word kLaneReportOffset.
+constexpr size_t kLanePinnedBytes = 256;
+constexpr size_t kLaneReportOffset = 32;
 
 sortStatus_t mgpu_lane_init(SortLane* l);
 void mgpu_lane_free(SortLane* l);

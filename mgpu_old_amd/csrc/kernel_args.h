@@ -100,6 +100,11 @@ struct mgpuScanArgs {
     uint32_t* seg_count;      // per-segment counts the passes fill in -> zeroed
     uint32_t seg_count_words;
     uint32_t pad_;
+    // Early exit (null: none): host-visible words the scan fills for the
+    // host's decision: [0] descents of the bit field, [1] descents of pass
+    // 0's digit (both from digit_count), [2 + p] the count of pass p's most
+    // frequent digit.
+    uint32_t* report;
 };
 
 // Counter words of a lane: pass p's tile counter of segment x, the

@@ -350,7 +350,10 @@ __device__ void digit_scan_body(const mgpuScanArgs& a) {
         c[x] = x < a.num_segs ? a.hist[(x * MGPU_MAX_PASSES + p) * RADIX + tid] : 0u;
         total += c[x];
     }
-    if (a.seg_count_words) {  // segments per XCD: the pass's most frequent digit
+    // The pass's most frequent digit: its value for segments per XCD, its
+    // count for the early-exit report (a pass whose digit holds every key is
+    // the identity).
+    if (a.seg_count_words || a.report) {
         const uint64_t best = wave_max_u64(((uint64_t)total << 8) | tid);
         if (lane == 0) s_max[wave] = best;
         __syncthreads();
@@ -358,7 +361,18 @@ __device__ void digit_scan_body(const mgpuScanArgs& a) {
             uint64_t m = s_max[0];
             for (int w = 1; w < 4; ++w) m = s_max[w] > m ? s_max[w] : m;
             a.tile_counters[MGPU_MODE_SLOT + p] = (uint32_t)(m & 0xFFu);
+            if (a.report) {
+                __hip_atomic_store(&a.report[2 + p], (uint32_t)(m >> 8), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_SYSTEM);
+                if (p == 0) {
+                    __hip_atomic_store(&a.report[0], a.tile_counters[MGPU_UNSORTED_SLOT],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    __hip_atomic_store(&a.report[1], a.tile_counters[MGPU_UNSORTED_SLOT + 1],
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+            }
         }
+        __syncthreads();  // the counters' reads above come before block 0 clears them
     }
     uint32_t run = block_exclusive_scan<256>(total, s_tmp, lane, wave);
 #pragma unroll
