@@ -178,6 +178,19 @@ sortStatus_t SORTAPI sortFreeData(sortData_d* data);
  * the sort server reports it per batch in sortCompletion_d.status. */
 sortStatus_t SORTAPI sortArray(sortEngine_t engine, sortData_t data);
 
+/* No counterpart in the reference.  sortArray for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments, kernels, result, parity
+ * and error reporting as sortArray, but no hipMalloc, hipFree, memset or
+ * host synchronization: the scratch is a fixed array in the library's device
+ * code (about 8.2 MB per GPU, held until the process ends).  At most
+ * SORT_NOALLOC_MAX_ELEMENTS elements (at the engine's default tile shapes;
+ * more returns SORT_STATUS_CONFIG_NOT_SUPPORTED); earlyExit is ignored (it
+ * needs a blocking readback).  Every call on one GPU shares that array, so
+ * calls that overlap on the same GPU race: one at a time per device. */
+#define SORT_NOALLOC_MAX_ELEMENTS (1 << 24)
+sortStatus_t SORTAPI sortArrayNoAlloc(sortEngine_t engine, sortData_t data);
+
 /* Replaces inc/mgpusort.h:169-170.  Same as sortArray with the pass split
  * chosen by the caller: ceil(bits / bitPass) passes of bitPass bits (the first
  * ones a bit narrower when it does not divide), bitPass outside 1..6 taken as

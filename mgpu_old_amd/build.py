@@ -29,8 +29,9 @@ CODE_OBJECTS = {
     "mgpusort_seg.hsaco": ["segsort.hip"],
 }
 HOST_SOURCES = ["engine.cpp", "loader.cpp", "segmented.cpp", "server.cpp"]
-# Host code with device code of its own, linked into libmgpusort.so.
-HIP_HOST_SOURCES = []
+# Host code with device code of its own, linked into libmgpusort.so: the
+# fixed scratch array of sortArrayNoAlloc (and its clear kernel).
+HIP_HOST_SOURCES = ["noalloc.hip"]
 
 
 def _run(cmd):

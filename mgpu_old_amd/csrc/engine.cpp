@@ -611,6 +611,9 @@ sortStatus_t SORTAPI sortCreateEngine(const char* kernelPath, sortEngine_t* engi
         mgpu_engine_destroy(e);
         return st;
     }
+    // sortArrayNoAlloc's scratch on this device (its absence only disables
+    // that entry point).
+    (void)mgpu_noalloc_setup(device);
     *engine = e;
     return SORT_STATUS_SUCCESS;
 }

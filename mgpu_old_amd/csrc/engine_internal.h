@@ -160,6 +160,10 @@ struct sortEngine_d {
     std::map<hipFunction_t, std::string> kernelNames;
 };
 
+// Looks up the current device's copy of sortArrayNoAlloc's scratch array
+// once per device (false: not available on this device).
+bool mgpu_noalloc_setup(int device);
+
 // Loads the code objects named from e->kernelPath and allocates the fixed
 // scratch.  On failure the engine is left for mgpu_engine_destroy.
 sortStatus_t mgpu_engine_load(sortEngine_d* e);
