@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--no-segmented", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-early-exit", action="store_true")
+    ap.add_argument("--segmented-direct", action="store_true",
+                    help="segmented leg through sortSegmentedArray instead of plan replays")
     ap.add_argument("--no-headline", action="store_true",
                     help="skip the headline sort (profiling the other legs alone); value is null")
     ap.add_argument("--no-cpu", action="store_true")
@@ -259,9 +261,13 @@ def early_exit_bench(torch, eng, log2n, steps):
     return out
 
 
-def segmented_bench(torch, eng, steps):
+def segmented_bench(torch, eng, steps, direct=False):
     """4096 segments, sizes log-uniform in [1, 2^16], key + one value array:
-    graph-replayed segmented plan vs a loop of sortArray per segment."""
+    graph-replayed segmented plan vs a loop of sortArray per segment.  With
+    direct, sortSegmentedArray calls (each synchronous, no graph) take the
+    plan's place and the loop is left out: the form for rocprofv3's counter
+    mode, which follows no graph replays and crashed inside the loop's
+    4096 sortArray launches."""
     import mgpu_old_amd as mg
     from mgpu_old_amd import workloads
     sizes = workloads.log_uniform_sizes(4096, 0, 16, seed=4)
@@ -280,17 +286,33 @@ def segmented_bench(torch, eng, steps):
         d.valueCount, d.firstBit, d.endBit = 1, 0, 32
         d.keys[0], d.keys[1] = k.data_ptr(), k1.data_ptr()
         d.values[0][0], d.values[0][1] = v.data_ptr(), v1.data_ptr()
-        copies.append((d, k, v, k1, v1, eng.segmented_plan(d, starts)))
+        copies.append((d, k, v, k1, v1, None if direct else eng.segmented_plan(d, starts)))
     torch.cuda.synchronize()
-    copies[0][5].execute(copies[0][0])  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for d, *_rest, plan in copies[1:]:
-        plan.execute(d)
-    torch.cuda.synchronize()
+    if direct:
+        eng.sort_segmented(copies[0][0], starts)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for d, *_rest in copies[1:]:
+            eng.sort_segmented(d, starts)
+        torch.cuda.synchronize()
+    else:
+        copies[0][5].execute(copies[0][0])  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for d, *_rest, plan in copies[1:]:
+            plan.execute(d)
+        torch.cuda.synchronize()
     seg_s = (time.perf_counter() - t0) / steps
     eng.check_device()
     seg_k, seg_v = copies[1][1], copies[1][2]  # 4 passes: result back in side 0
+
+    if direct:
+        out = {"segments": 4096, "n": n, "form": "direct", "ms_segmented": seg_s * 1e3,
+               "gkeys_per_s_segmented": n / seg_s / 1e9}
+        log(f"[bench] segmented (direct): {out['ms_segmented']:.3f} ms")
+        del copies
+        torch.cuda.empty_cache()
+        return out
 
     # Baseline: one sortArray per segment on the same input.
     k, v = keys.clone(), vals.clone()
@@ -313,8 +335,10 @@ def segmented_bench(torch, eng, steps):
     eng.check_device()
     same = bool(torch.equal(seg_k, k)) and bool(torch.equal(seg_v, v))
     for c in copies:
-        c[5].close()
-    out = {"segments": 4096, "n": n, "ms_segmented": seg_s * 1e3,
+        if c[5] is not None:
+            c[5].close()
+    out = {"segments": 4096, "n": n, "form": "plan replay",
+           "ms_segmented": seg_s * 1e3,
            "gkeys_per_s_segmented": n / seg_s / 1e9, "ms_sortarray_loop": loop_s * 1e3,
            "gkeys_per_s_sortarray_loop": n / loop_s / 1e9, "speedup_vs_loop": loop_s / seg_s,
            "bit_exact_vs_loop": same}
@@ -526,7 +550,7 @@ def main():
         e.pop("kernels", None)
 
     ee = None if args.no_early_exit else early_exit_bench(torch, eng, args.log2n, 5)
-    seg = None if args.no_segmented else segmented_bench(torch, eng, 10)
+    seg = None if args.no_segmented else segmented_bench(torch, eng, 10, args.segmented_direct)
     served = None if args.no_served else served_bench(torch, eng, args.served_requests)
 
     cpu, cpu_runs = None, None

@@ -59,6 +59,11 @@ __device__ __forceinline__ uint32_t prev_lane(uint32_t v, uint32_t lane0) {
     return (uint32_t)__builtin_amdgcn_update_dpp((int)lane0, (int)v, 0x138, 0xF, 0xF, false);
 }
 
+// Lane i gets v of lane i + 1, lane 63 gets lane63 (DPP wave_shl:1).
+__device__ __forceinline__ uint32_t next_lane(uint32_t v, uint32_t lane63) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)lane63, (int)v, 0x130, 0xF, 0xF, false);
+}
+
 // Orders this wave's LDS accesses before and after the call.
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -218,17 +223,24 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
     // Early-exit check: descents between neighbours (each key against the one
     // after it, across chunk ends too) of the sort's bit field and of the
     // first pass's digit (what the reference's per-pass count checks: the
-    // whole key and the current digit).  The key after the last of a lane's
-    // vector is the first of the next lane's vector (a shuffle), read from
-    // memory only for the last lane and at the end of the vectors.
+    // whole key and the current digit).  In the vector body a wave reads 64
+    // consecutive vectors per step, so the key after a lane's vector is the
+    // next lane's first (DPP), or for lane 63 the first of lane 0's vector
+    // of the next step (readlane), or after the wave's last step the next
+    // wave's first key, which lane 63 loads with the round's vectors.  No
+    // lane waits on memory inside the loop.  The pair that joins the vector
+    // body to the keys after it is taken once, after the loop.
+    // Fields are compared in place: x & M > y & M orders two keys as their
+    // extracted fields would (both sit at the same bit position).
     const bool check = a.unsorted != nullptr;
-    const uint32_t cs = a.check_shift, cm = a.check_mask;
-    uint32_t descents = 0, first_desc = 0;
-    auto bf = [&](uint32_t k) { return (k >> cs) & cm; };
+    const uint32_t mfull = a.check_mask << a.check_shift, mfirst = a.mask[0] << a.shift[0];
+    uint32_t descents = 0, first_desc = 0;  // per lane (scalar tails)
+    uint32_t wdesc = 0, wfirst = 0;         // per wave, by ballot (vector body)
     auto pair = [&](uint32_t x, uint32_t y) {  // x just before y in the input
-        descents += bf(x) > bf(y);
-        first_desc += ((x >> a.shift[0]) & a.mask[0]) > ((y >> a.shift[0]) & a.mask[0]);
+        descents += (x & mfull) > (y & mfull);
+        first_desc += (x & mfirst) > (y & mfirst);
     };
+    auto ballots = [](bool p) { return (uint32_t)__builtin_popcountll(__builtin_amdgcn_ballot_w64(p)); };
     if (begin < end) {
         if (a.vec_ok) {
             typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -239,44 +251,60 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
             // (the one-workgroup-per-CU column kernel: 8, 0.23 -> 0.21 ms at
             // 2^28; 16 spills).
             constexpr int UNROLL = COLS ? 8 : 4;
-            for (uint32_t i0 = tid; i0 < nvec; i0 += UNROLL * BLOCK) {
+            const uint32_t lane = tid & 63;
+            // Vectors of a round: wave w's step u covers the 64 vectors from
+            // round + (w * UNROLL + u) * 64.  Every lane runs the same rounds
+            // (lanes past the chunk's vectors are masked off), so that the
+            // descent counts can be taken by wave-wide ballots.
+            const uint32_t wbase = wave * 64 * UNROLL + lane;
+            for (uint32_t round = 0; round < nvec; round += UNROLL * BLOCK) {
+                const uint32_t i0 = round + wbase;
                 u32x4 q[UNROLL];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
-                    const uint32_t i = i0 + u * BLOCK;
-                    if (i < nvec) q[u] = __builtin_nontemporal_load(v + i);
+                    const uint32_t i = i0 + u * 64;
+                    q[u] = i < nvec ? __builtin_nontemporal_load(v + i) : u32x4{0u, 0u, 0u, 0u};
                 }
+                const uint32_t ilast = i0 + (UNROLL - 1) * 64;
+                uint32_t edge = 0;
+                if (check && lane == 63 && ilast + 1 < nvec) edge = a.keys[begin + 4 * ilast + 4];
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
-                    const uint32_t i = i0 + u * BLOCK;
-                    if (i >= nvec) break;
+                    const uint32_t i = i0 + u * 64;
+                    const bool valid = i < nvec;
+                    if (valid) {
 #pragma unroll
-                    for (int p = 0; p < MGPU_MAX_PASSES; ++p) {
-                        if ((uint32_t)p >= np) continue;
-                        if constexpr (COLS == 0) {
-                            count4(mine + p * PSTRIDE, q[u].x, q[u].y, q[u].z, q[u].w, a.shift[p],
-                                   a.mask[p]);
-                        } else {
-                            add(p, (q[u].x >> a.shift[p]) & a.mask[p]);
-                            add(p, (q[u].y >> a.shift[p]) & a.mask[p]);
-                            add(p, (q[u].z >> a.shift[p]) & a.mask[p]);
-                            add(p, (q[u].w >> a.shift[p]) & a.mask[p]);
+                        for (int p = 0; p < MGPU_MAX_PASSES; ++p) {
+                            if ((uint32_t)p >= np) continue;
+                            if constexpr (COLS == 0) {
+                                count4(mine + p * PSTRIDE, q[u].x, q[u].y, q[u].z, q[u].w,
+                                       a.shift[p], a.mask[p]);
+                            } else {
+                                add(p, (q[u].x >> a.shift[p]) & a.mask[p]);
+                                add(p, (q[u].y >> a.shift[p]) & a.mask[p]);
+                                add(p, (q[u].z >> a.shift[p]) & a.mask[p]);
+                                add(p, (q[u].w >> a.shift[p]) & a.mask[p]);
+                            }
                         }
                     }
                     if (check) {
-                        pair(q[u].x, q[u].y);
-                        pair(q[u].y, q[u].z);
-                        pair(q[u].z, q[u].w);
-                        // Vector i + 1 is in the next lane when that lane
-                        // holds one of this chunk's vectors (it is active).
-                        const uint32_t next = begin + 4 * i + 4;
-                        const uint32_t lane = tid & 63;
-                        const uint32_t nb = __shfl_down(q[u].x, 1, 64);
-                        if (lane < 63 && i + 1 < nvec) pair(q[u].w, nb);
-                        else if (next < a.n) pair(q[u].w, a.keys[next]);
+                        // The key after this vector (used only when vector
+                        // i + 1 is one of the chunk's: then it is in a lane).
+                        const uint32_t after63 =
+                            u + 1 < UNROLL ? __builtin_amdgcn_readlane(q[u + 1 < UNROLL ? u + 1 : u].x, 0)
+                                           : edge;
+                        const uint32_t nx = next_lane(q[u].x, after63);
+                        const bool has = i + 1 < nvec;
+                        const u32x4 f = q[u] & mfull, g = q[u] & mfirst;
+                        wdesc += ballots(valid && f.x > f.y) + ballots(valid && f.y > f.z) +
+                                 ballots(valid && f.z > f.w) + ballots(has && f.w > (nx & mfull));
+                        wfirst += ballots(valid && g.x > g.y) + ballots(valid && g.y > g.z) +
+                                  ballots(valid && g.z > g.w) + ballots(has && g.w > (nx & mfirst));
                     }
                 }
             }
+            if (check && tid == 0 && nvec > 0 && begin + 4 * nvec < a.n)
+                pair(a.keys[begin + 4 * nvec - 1], a.keys[begin + 4 * nvec]);
             for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) {
                 const uint32_t k = a.keys[i];
                 count_key(k);
@@ -298,7 +326,7 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
             for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
             return v;
         };
-        const uint32_t dsum = wave_sum(descents), fsum = wave_sum(first_desc);
+        const uint32_t dsum = wave_sum(descents) + wdesc, fsum = wave_sum(first_desc) + wfirst;
         if ((tid & 63) == 0) {
             if (dsum) __hip_atomic_fetch_add(&s_desc[0], dsum, __ATOMIC_RELAXED,
                                              __HIP_MEMORY_SCOPE_WORKGROUP);

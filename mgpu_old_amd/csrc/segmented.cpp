@@ -20,6 +20,12 @@
 
 namespace {
 
+// Segments whose device-wide tiles average at least this many take the
+// lookback chains per XCD (measured, key + value, replays: 64 segments
+// log-uniform in [2^18, 2^20] 0.734 -> 0.640 ms, 512 in [2^15, 2^17] 0.847 ->
+// 0.764 ms; 4096 in [1, 2^16], 2.7 tiles each, 0.673 -> 0.729 ms).
+constexpr uint64_t kXcdMinTiles = 4;
+
 // Segments of at least wide.minKeys keys go to the device-wide passes (all
 // of them together, each spread over many workgroups with a lookback chain)
 // instead of one workgroup each.  Above 4096 keys (the largest a 256-thread
@@ -35,15 +41,24 @@ uint32_t device_wide_min(const sortEngine_d* e) {
     return x > 0 ? (uint32_t)x : e->segWide.minKeys;
 }
 // Keys per digit_count workgroup on that path (a multiple of 1024 up to 2^20,
-// where the 16-bit lane-column counts of the count kernel would start to
-// overflow; MGPUSORT_SEG_WIDE_CHUNK overrides the table): at the default
-// 2^16 a segment of up to 2^16 keys is one piece, so its histogram is merged
-// into global memory once.
+// where the 16-bit lane-column counts of the 1024-thread count kernel would
+// start to overflow; MGPUSORT_SEG_WIDE_CHUNK overrides the table).
 uint32_t clamp_chunk(long x) { return (uint32_t)std::min<long>(std::max<long>(x, 1024), 1L << 20) & ~1023u; }
 uint32_t wide_chunk(const sortEngine_d* e) {
     const char* v = std::getenv("MGPUSORT_SEG_WIDE_CHUNK");
     const long x = v ? std::strtol(v, nullptr, 10) : 0;
     return x >= 1024 ? clamp_chunk(x) : e->segWide.chunk;
+}
+// Count kernel of the device-wide passes: lane-column histograms in
+// 1024-thread workgroups (default), or per-wave histograms in 256-thread
+// workgroups with MGPUSORT_SEG_COUNT=wave.  Measured by plan replays, key +
+// value, per-wave -> lane-column: 4096 segments log-uniform in [1, 2^16]
+// 0.674 -> 0.666 ms; 512 in [2^15, 2^17] 0.841 -> 0.781 ms with one chain per
+// segment and 0.755 -> 0.695 ms with chains per XCD; 64 in [2^18, 2^20]
+// 0.639 -> 0.645 ms (chains per XCD).
+bool count_cols() {
+    const char* v = std::getenv("MGPUSORT_SEG_COUNT");
+    return !(v && std::strcmp(v, "wave") == 0);
 }
 
 const SegTuneEntry kDefaultTable[] = {
@@ -292,9 +307,12 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         // alone holds its chain (~0.45 us a round trip instead of ~0.85 us
         // through the device-coherent level).  Segments go to the XCD with
         // the fewest tiles so far, largest first, and are then laid out XCD
-        // by XCD; only when no segment outweighs a quarter of an XCD's share
-        // (else one XCD would do most of the work, and one chain across the
-        // device is kept).
+        // by XCD.  Only for segments of longer chains (at least kXcdMinTiles
+        // tiles on average: short chains wait little, and the workgroups
+        // that loop over an XCD's tiles pay a counter round trip and a
+        // barrier per tile), and when no segment outweighs a quarter of an
+        // XCD's share (else one XCD would do most of the work).
+        // MGPUSORT_SEG_XCD=0|1 forces the choice.
         P.wideXcds = 1;
         P.xcdTileBegin.clear();
         const uint32_t X = e->xcdSegs;
@@ -305,7 +323,11 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
                 total += t;
                 most = std::max(most, t);
             }
-            if (most * 4 * X <= total) {
+            const char* force = std::getenv("MGPUSORT_SEG_XCD");
+            bool use = most * 4 * X <= total && total >= kXcdMinTiles * P.wide.size();
+            if (force && (force[0] == '0' || force[0] == '1') && force[1] == 0)
+                use = force[0] == '1' && most * 4 * X <= total;
+            if (use) {
                 std::vector<uint32_t> order(P.wide);
                 std::stable_sort(order.begin(), order.end(),
                                  [&](uint32_t a, uint32_t b) { return segs[a].n > segs[b].n; });
@@ -481,14 +503,18 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         ca.mask[p] = (1u << P.sortWidths[p]) - 1u;
         bit += P.sortWidths[p];
     }
-    // One 1024-thread workgroup per piece (two fit on a CU), enough of them
-    // to zero the lookback words quickly as well.
+    // One workgroup per piece, enough of them to zero the lookback words
+    // quickly as well.
+    const bool cols = count_cols();
+    const uint32_t threads = cols ? 1024u : 256u;
     uint32_t grid = P.wideChunks;
     const uint32_t clearGrid = (uint32_t)std::min<uint64_t>(
-        (ca.clear_words / 4 + 1024 * 16 - 1) / (1024 * 16), (uint64_t)e->numCUs * 2);
+        (ca.clear_words / 4 + threads * 16 - 1) / (threads * 16),
+        (uint64_t)e->numCUs * (cols ? 2 : 8));
     grid = std::max(grid, clearGrid);
     if (!P.ownMulti) lane.multiDirty = true;
-    sortStatus_t st = mgpu_launch(e, lane.stream, e->fnMultiCountCols, grid, 1024, &ca);
+    sortStatus_t st = mgpu_launch(e, lane.stream, cols ? e->fnMultiCountCols : e->fnMultiCount,
+                                  grid, threads, &ca);
     if (st) return st;
 
     mgpuMultiScanArgs sa;

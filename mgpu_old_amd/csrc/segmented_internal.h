@@ -47,7 +47,7 @@ struct SegTuneEntry {
 // (0: small unless segments average four large tiles, 1 small, 2 large).
 struct SegWideTune {
     uint32_t minKeys = 4096 + 1;
-    uint32_t chunk = 1u << 16;
+    uint32_t chunk = 1u << 14;
     int tile = 0;
 };
 

@@ -214,21 +214,27 @@ def test_index_mode_needs_both_value_sides(engine, eb):
                                      ctypes.byref(plan)) == mg.SortStatus.INVALID_VALUE
 
 
-@pytest.mark.parametrize("layout", ["xcd", "one_chain", "small_pieces", "dominant_segment"])
+@pytest.mark.parametrize("layout", ["xcd", "one_chain", "small_pieces", "wave_count",
+                                    "dominant_segment"])
 def test_device_wide_layouts(oracle, layout):
     """The device-wide passes of a segmented sort in each layout: lookback
-    chains per XCD with whole segments grouped by XCD (default), one chain
-    per segment across the device (MGPUSORT_XCD=0, or a segment holding more
-    than a quarter of an XCD's share of the tiles), and count pieces smaller
-    than a segment (MGPUSORT_SEG_WIDE_CHUNK=4096).  Every segment bit-exact
+    chains per XCD with whole segments grouped by XCD (MGPUSORT_SEG_XCD=1),
+    one chain per segment across the device (MGPUSORT_XCD=0; or a segment
+    holding more than a quarter of an XCD's share of the tiles, even when
+    asked for chains per XCD), count pieces smaller than a segment
+    (MGPUSORT_SEG_WIDE_CHUNK=4096, one chain per segment:
+    MGPUSORT_SEG_XCD=0), and the per-wave count kernel instead of the
+    lane-column one (MGPUSORT_SEG_COUNT=wave).  Every segment bit-exact
     against the oracle."""
     import torch
     env = {"MGPUSORT_XCD": "0"} if layout == "one_chain" else {}
     eng = make_engine(env)
-    saved = os.environ.get("MGPUSORT_SEG_WIDE_CHUNK")
+    knobs = {"xcd": {"MGPUSORT_SEG_XCD": "1"}, "dominant_segment": {"MGPUSORT_SEG_XCD": "1"},
+             "small_pieces": {"MGPUSORT_SEG_WIDE_CHUNK": "4096", "MGPUSORT_SEG_XCD": "0"},
+             "wave_count": {"MGPUSORT_SEG_COUNT": "wave"}}.get(layout, {})
+    saved = {k: os.environ.get(k) for k in knobs}
     try:
-        if layout == "small_pieces":
-            os.environ["MGPUSORT_SEG_WIDE_CHUNK"] = "4096"
+        os.environ.update(knobs)
         sizes = list(workloads.log_uniform_sizes(300, 12, 17, seed=320))
         if layout == "dominant_segment":
             sizes[7] = 1 << 22
@@ -238,13 +244,24 @@ def test_device_wide_layouts(oracle, layout):
         vals = [workloads.uniform_keys(n, seed=322)]
         for vc, fb, eb in [(1, 0, 32), (-1, 3, 27)]:
             vv = vals if vc == 1 else []
+            eng.enable_timing(True)
+            eng.read_timing()
             g = _run_segmented(eng, keys, vv, vc, starts, fb, eb)
+            names = set(eng.read_timing())
+            eng.enable_timing(False)
             _check_segments(oracle, keys, vv, vc, starts, g.keys(), g.values(), fb, eb)
+            per_xcd = any(k.startswith("mgpu_multi_scatter") and "_x_" in k for k in names)
+            if layout == "xcd" and eng.info()["xcdSegments"] > 1:
+                assert per_xcd, names
+            if layout in ("one_chain", "dominant_segment", "small_pieces"):
+                assert not per_xcd, names
+            assert ("mgpu_multi_count_cols" in names) == (layout != "wave_count"), names
     finally:
-        if saved is None:
-            os.environ.pop("MGPUSORT_SEG_WIDE_CHUNK", None)
-        else:
-            os.environ["MGPUSORT_SEG_WIDE_CHUNK"] = saved
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
         eng.close()
 
 
