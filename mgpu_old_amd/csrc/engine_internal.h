@@ -22,6 +22,11 @@ struct PassVariant {
     hipFunction_t multi[4] = {};  // mgpu_multi_scatter_b*_i*_r*_v<mode>
     hipFunction_t multiX[4] = {}; // mgpu_multi_scatter_b*_i*_r*_x_v<mode>: chains per XCD
     uint32_t multiXGrid = 0;      // workgroups of a multiX launch: all that fit
+    // Packed tiles of several arrays (atomic-add ranking only; null when not
+    // built for the shape): mgpu_multi_packed_b*_i*_v<mode> and ..._x_v<mode>.
+    hipFunction_t packed[4] = {};
+    hipFunction_t packedX[4] = {};
+    uint32_t packedXGrid = 0;
     // Early-exit forms (atomic-add ranking only; null when not built for the
     // shape): mgpu_rank_scatter_b*_i*_r1_ee_v<mode> and ..._r1_x_ee_v<mode>.
     hipFunction_t passEE[4] = {};

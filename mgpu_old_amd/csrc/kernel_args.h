@@ -201,6 +201,13 @@ struct mgpuMultiCountArgs {
     uint32_t num_passes;
     uint32_t shift[MGPU_MAX_PASSES];
     uint32_t mask[MGPU_MAX_PASSES];
+    // Lane-column kernel only: every chunk is a whole array (direct != 0),
+    // so the kernel writes the digit starts and zeroes the tile counters
+    // itself, in place of multi_scan; hist is not used.
+    uint32_t* digit_start;    // [num_segs][MGPU_MAX_PASSES][MGPU_RADIX]
+    uint32_t* tile_counters;  // [MGPU_LANE_COUNTERS]
+    uint32_t direct;
+    uint32_t pad_;
 };
 
 // digit_scan, one workgroup per array; zeroes hist and the tile counters.
@@ -237,7 +244,21 @@ struct mgpuMultiPassArgs {
     uint32_t* finished;
     uint32_t num_xcds;
     uint32_t pad_;
+    // Packed tiles (mgpu_multi_packed_* kernels only): the arrays are cut into
+    // wave chunks of 64 * IPT keys, laid out back to back (an array's last
+    // chunk may be partial), and a tile is BLOCK / 64 consecutive chunks, so
+    // one tile can hold the end of one array, whole small arrays and the
+    // start of another ("parts", at most MGPU_PACK_MAX_PARTS(W) per tile).
+    const uint32_t* chunk_begin;  // [num_segs]: first chunk of each array
+    // [num_tiles][2]: word 0 = the array of the tile's first chunk; word 1 =
+    // bits 0-15: bit w set when chunk w starts a new array (bit 0 clear),
+    // bits 16-23: chunks holding keys (the rest are padding), bit 24: the
+    // first chunk continues an array begun in an earlier tile.
+    const uint32_t* tile_info;
 };
+
+// Parts (arrays) per packed tile of W waves.
+#define MGPU_PACK_MAX_PARTS(W) ((W) / 2)
 
 // rank_scatter: one stable digit pass (rank in the tile, decoupled lookback
 // across tiles, LDS-staged scatter).  With num_segs > 1 the workgroups of XCD

@@ -62,6 +62,26 @@ sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, uin
             e->kernelNames[f] = name;
         }
     }
+    // Packed multi-array forms, likewise.
+    bool packed = rank == 1;
+    for (int m = 0; m < 4 && packed; ++m) {
+        char name[96];
+        hipFunction_t f = nullptr, fx = nullptr;
+        std::snprintf(name, sizeof(name), "mgpu_multi_packed_b%u_i%u_v%d", threads, items, m);
+        char namex[96];
+        std::snprintf(namex, sizeof(namex), "mgpu_multi_packed_b%u_i%u_x_v%d", threads, items, m);
+        if (hipModuleGetFunction(&f, e->radixModule, name) != hipSuccess ||
+            hipModuleGetFunction(&fx, e->radixModule, namex) != hipSuccess) {
+            packed = false;
+            break;
+        }
+        v->packed[m] = f;
+        v->packedX[m] = fx;
+        e->kernelNames[f] = name;
+        e->kernelNames[fx] = namex;
+    }
+    if (!packed)
+        for (int m = 0; m < 4; ++m) v->packed[m] = v->packedX[m] = nullptr;
     (void)hipGetLastError();
     // Segment workgroups loop over tiles: launch as many as fit at once.
     int perCU = 0;
@@ -76,6 +96,14 @@ sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, uin
         perCU < 1)
         perCU = 1;
     v->multiXGrid = (uint32_t)perCU * (uint32_t)e->numCUs;
+    if (v->packedX[MGPU_VALUES_SINGLE]) {
+        perCU = 0;
+        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(
+                &perCU, v->packedX[MGPU_VALUES_SINGLE], (int)threads, 0) != hipSuccess ||
+            perCU < 1)
+            perCU = 1;
+        v->packedXGrid = (uint32_t)perCU * (uint32_t)e->numCUs;
+    }
     return SORT_STATUS_SUCCESS;
 }
 

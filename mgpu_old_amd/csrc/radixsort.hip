@@ -111,6 +111,25 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t x, uint32_t* s
     return before + inc - x;
 }
 
+// Exclusive scan of x within each group of RADIX consecutive threads (RADIX /
+// 64 waves).  s_tmp holds one word per wave.  Contains barriers: every thread
+// of the block must call it.
+template <int BLOCK>
+__device__ __forceinline__ uint32_t group_exclusive_scan(uint32_t x, uint32_t* s_tmp,
+                                                         uint32_t lane, uint32_t wave) {
+    constexpr uint32_t GW = RADIX / 64;  // waves per group
+    const uint32_t inc = wave_inclusive_scan(x, lane);
+    if (lane == 63) s_tmp[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0;
+    const uint32_t g0 = wave & ~(GW - 1u);
+#pragma unroll
+    for (uint32_t w = 0; w < GW; ++w)
+        if (g0 + w < wave) before += s_tmp[g0 + w];
+    __syncthreads();
+    return before + inc - x;
+}
+
 // Adds 4 keys per lane (16-byte loads) to a per-wave LDS histogram of one
 // pass's digit.  Sorted and constant input put every lane on one counter, and
 // an LDS atomic serializes the lanes that hit the same word (2^28 sorted keys:
@@ -548,15 +567,25 @@ __device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
     // Thread (p, d) = (tid / 256, tid % 256) sums its digit's 32 columns,
     // starting at column tid so that a half-wave reads 32 distinct banks.
     const uint32_t p = tid / RADIX, d = tid % RADIX;
+    uint32_t sum = 0;
     if (p < np) {
         const uint32_t row = (p * RADIX + d) >> 1, sh = (d & 1u) << 4;
-        uint32_t sum = 0;
 #pragma unroll
         for (uint32_t c = 0; c < COLS; ++c)
             sum += (s_hist[row * COLS + ((c + tid) & (COLS - 1))] >> sh) & 0xFFFFu;
-        if (sum)
-            __hip_atomic_fetch_add(&a.hist[((size_t)sg * MGPU_MAX_PASSES + p) * RADIX + d], sum,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (a.direct) {
+        // The piece is the whole array: its digit starts, pass by pass (a
+        // scan per group of 256 threads), with no global histogram and no
+        // separate scan launch.  Workgroup 0 zeroes the passes' tile
+        // counters, as that launch would.
+        __shared__ uint32_t s_tmp[BLOCK / 64];
+        const uint32_t ex = group_exclusive_scan<BLOCK>(sum, s_tmp, tid & 63, tid >> 6);
+        if (p < np) a.digit_start[((size_t)sg * MGPU_MAX_PASSES + p) * RADIX + d] = ex;
+        if (blockIdx.x == 0 && tid < MGPU_LANE_COUNTERS) a.tile_counters[tid] = 0;
+    } else if (sum) {
+        __hip_atomic_fetch_add(&a.hist[((size_t)sg * MGPU_MAX_PASSES + p) * RADIX + d], sum,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -1194,6 +1223,355 @@ __device__ void rank_scatter_body(const Args& a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Packed device-wide passes of a segmented sort (mgpu_multi_packed_*).  The
+// arrays are cut into wave chunks of 64 * IPT keys and laid out back to back,
+// BLOCK / 64 chunks to a tile, so an array's last, partial chunk shares its
+// tile with the arrays after it instead of leaving the rest of a tile empty
+// (the tile-per-array layout of rank_scatter_body<MULTI> ran 3460 tiles for
+// the keys of 2783 on 4096 segments log-uniform in [1, 2^16]).  A run of
+// chunks of one array in a tile is a part.  Each wave ranks its chunk as in
+// rank_scatter_body (RANK 1); the tile is staged part by part, each part in
+// digit order, and written out in rounds of BLOCK consecutive positions,
+// each inside one chunk-sized stretch of the stage and so inside one part:
+// the array a round writes to is the same for every lane.  Only a tile whose
+// first chunk continues an array begun earlier walks back along the chain.
+// Every tile publishes the digit counts of its last part: as a prefix when
+// that array starts in the tile (nothing before it counts), else as an
+// aggregate, and as a prefix once its own walk has ended.
+template <int BLOCK, int IPT, int VMODE, bool XSEG>
+__device__ void packed_scatter_body(const mgpuMultiPassArgs& a) {
+    constexpr int W = BLOCK / 64;
+    constexpr uint32_t WT = 64 * IPT;  // keys of a wave chunk
+    constexpr uint32_t TILE = BLOCK * IPT;
+    constexpr uint32_t PMAX = MGPU_PACK_MAX_PARTS(W);
+    constexpr uint32_t GROUPS = BLOCK / RADIX;  // parts scanned at once
+    static_assert(BLOCK % RADIX == 0 && BLOCK <= (int)WT && WT % BLOCK == 0,
+                  "a write-out round must lie inside one chunk");
+    static_assert(PMAX >= 1, "at least one part per tile");
+    constexpr bool JOINT = VMODE == MGPU_VALUES_SINGLE || VMODE == MGPU_VALUES_INDEX;
+    __shared__ uint32_t s_stage[TILE];
+    __shared__ uint32_t s_vstage[JOINT ? TILE : 1];
+    __shared__ uint32_t s_count[W][RADIX];
+    // Per (part, digit): the part's digit total, then its first staging
+    // position, then (output position - staging position).
+    __shared__ uint32_t s_part[PMAX][RADIX];
+    __shared__ uint32_t s_prefix[RADIX];  // first part: its digits in earlier tiles
+    __shared__ uint32_t s_last[RADIX];    // last part: its digit counts in this tile
+    __shared__ uint32_t s_valid[W];       // keys of each chunk
+    __shared__ uint32_t s_pfirst[PMAX];   // first chunk of each part
+    __shared__ uint32_t s_pend[PMAX];     // end of each part's keys in the stage
+    __shared__ uint32_t s_tmp[W];
+    __shared__ uint32_t s_tile;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t shift = a.shift, mask = a.mask;
+    const uint32_t width = (uint32_t)__builtin_popcount(mask);  // mask = 2^width - 1
+    const uint32_t in = a.in_side, out = a.in_side ^ 1u;
+
+    // XSEG: this XCD's range of tiles (whole arrays), else every tile.
+    uint32_t xslot = 0, xfirst = 0, xtiles = a.num_tiles;
+    if constexpr (XSEG) {
+        uint32_t xcc;
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(xcc));
+        xslot = xcc & 0xFu;
+        if (xslot < a.num_xcds) {
+            xfirst = a.xcd_tile_begin[xslot];
+            xtiles = a.xcd_tile_begin[xslot + 1] - xfirst;
+        } else {
+            xtiles = 0;
+        }
+    }
+    for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&s_count[0][0])[i] = 0;
+
+    while (true) {
+        if (tid == 0) {
+            uint32_t t = xtiles;
+            if (xtiles > 0)
+                t = __hip_atomic_fetch_add(a.tile_counter + xslot, 1u, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+            s_tile = t;
+        }
+        __syncthreads();
+        const uint32_t lt = __builtin_amdgcn_readfirstlane(s_tile);
+        if (lt >= xtiles) {
+            if (!XSEG && tid == 0) set_error(a.error_flag, 2u);
+            break;
+        }
+        const uint32_t tile = xfirst + lt;  // global tile: its lookback words
+        const uint32_t first_arr = a.tile_info[2 * tile];
+        const uint32_t info = a.tile_info[2 * tile + 1];
+        const uint32_t starts = info & 0xFFFEu, live = (info >> 16) & 0xFFu;
+        const bool cont = ((info >> 24) & 1u) != 0;
+        const uint32_t nparts = (uint32_t)__builtin_popcount(starts) + 1u;
+        if (nparts > PMAX) {  // the host bounds the parts of every tile
+            if (tid == 0) set_error(a.error_flag, 6u);
+            break;
+        }
+        // Part of chunk w: the arrays that start at chunks 1..w.
+        auto part_of = [&](uint32_t w) -> uint32_t {
+            return (uint32_t)__builtin_popcount(starts & ((2u << w) - 1u));
+        };
+
+        // This wave's chunk: keys and the single value array, striped.
+        // Padding chunks (past `live`) read nothing: every slot is a filler.
+        const mgpuSegDesc& sd = a.segs[first_arr + part_of(wave)];
+        const uint32_t n = sd.n;
+        const bool has = wave < live;
+        const uint32_t wbase =
+            has ? (tile * W + wave - a.chunk_begin[first_arr + part_of(wave)]) * WT : n;
+        uint32_t key[IPT], val[IPT];
+        load_striped<IPT>(sd.keys[in], n, wbase, lane, key);
+        if constexpr (VMODE == MGPU_VALUES_SINGLE) load_striped<IPT>(sd.vals[0][in], n, wbase, lane, val);
+        // Fillers past the array's end get every bit set: the pass's top digit,
+        // ranked after every real key of their part.
+        const int32_t rem = (int32_t)(n - (wbase + lane));
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) key[j] = j * 64 < rem ? key[j] : 0xFFFFFFFFu;
+        auto digit_of = [&](int j) -> uint32_t { return __builtin_amdgcn_ubfe(key[j], shift, width); };
+
+        // Rank within the wave (atomic adds, served in ascending lane order).
+        uint32_t pos[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t d = digit_of(j);
+            const uint32_t f = __builtin_amdgcn_readfirstlane(d);
+            if (__builtin_amdgcn_ballot_w64(d == f) == ~0ull) {
+                uint32_t b = 0;
+                if (lane == 0)
+                    b = __hip_atomic_fetch_add(&s_count[wave][f], 64u, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                pos[j] = __builtin_amdgcn_readfirstlane(b) + lane;
+            } else {
+                pos[j] = __hip_atomic_fetch_add(&s_count[wave][d], 1u, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+        }
+        if (lane == 0) s_valid[wave] = has ? min(WT, n - wbase) : 0u;
+        __syncthreads();
+
+        // Per digit (thread d): offsets of each wave among the tile's keys of
+        // d, and each part's total of d (a difference of the offsets at the
+        // part's first chunk).  The last part's count (fillers taken off the
+        // top digit) is published at once.  Thread RADIX (thread 0 when the
+        // block has no more threads than digits): the parts' first chunks
+        // and stage ends.
+        uint32_t total = 0;
+        if (tid < RADIX) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                const uint32_t c = s_count[w][tid];
+                s_count[w][tid] = total;
+                total += c;
+            }
+            uint32_t m = starts, before = 0, q = 0;
+            while (m) {
+                const uint32_t w = (uint32_t)__builtin_ctz(m);
+                m &= m - 1u;
+                const uint32_t b = s_count[w][tid];
+                s_part[q][tid] = b - before;
+                before = b;
+                ++q;
+            }
+            if (nparts > 1) s_part[q][tid] = total - before;
+            uint32_t last = total - before;
+            if (tid == mask) {
+                const uint32_t lw = starts ? 31u - (uint32_t)__builtin_clz(starts) : 0u;
+                for (uint32_t w = lw; w < (uint32_t)W; ++w) last -= WT - s_valid[w];
+            }
+            s_last[tid] = last;
+            lookback_publish<XSEG>(&a.lookback[(size_t)tile * RADIX + tid],
+                                   (nparts == 1 && cont ? MGPU_LB_AGGREGATE : MGPU_LB_PREFIX) | last);
+        }
+        if (tid == (BLOCK > (int)RADIX ? RADIX : 0u)) {
+            uint32_t q = 0, w0 = 0, keys = 0;
+            for (uint32_t w = 0; w < (uint32_t)W; ++w) {
+                if (w > 0 && ((starts >> w) & 1u)) {
+                    s_pfirst[q] = w0;
+                    s_pend[q] = w0 * WT + keys;
+                    ++q;
+                    w0 = w;
+                    keys = 0;
+                }
+                keys += s_valid[w];
+            }
+            s_pfirst[q] = w0;
+            s_pend[q] = w0 * WT + keys;
+        }
+
+        if (nparts == 1) {
+            // One part, as in rank_scatter_body: the tile's digit starts (a
+            // block scan of the totals) added to the wave offsets.
+            const uint32_t start = block_exclusive_scan<BLOCK>(total, s_tmp, lane, wave);
+            if (tid < RADIX) {
+                s_part[0][tid] = start;
+#pragma unroll
+                for (int w = 0; w < W; ++w) s_count[w][tid] += start;
+            }
+        } else {
+            __syncthreads();
+            // Each part's digit starts in the stage: its first chunk's
+            // position plus the exclusive scan of its digit totals (GROUPS
+            // parts a round).
+            for (uint32_t pb = 0; pb < nparts; pb += GROUPS) {
+                const uint32_t q = pb + tid / RADIX, d = tid % RADIX;
+                const uint32_t v = q < nparts ? s_part[q][d] : 0u;
+                const uint32_t ex = group_exclusive_scan<BLOCK>(v, s_tmp, lane, wave);
+                if (q < nparts) s_part[q][d] = s_pfirst[q] * WT + ex;
+            }
+            __syncthreads();
+            // Wave offsets become stage positions: within its part a wave's
+            // keys of d follow the part's earlier waves' (offset at w minus
+            // offset at the part's first chunk, then the part's start of d).
+            // Waves in descending order, so a part's first-chunk offset is
+            // read before it is replaced.
+            if (tid < RADIX) {
+#pragma unroll
+                for (int w = W - 1; w >= 0; --w) {
+                    const uint32_t q = part_of((uint32_t)w);
+                    const uint32_t fw =
+                        31u - (uint32_t)__builtin_clz((starts | 1u) & ((2u << w) - 1u));
+                    s_count[w][tid] = s_count[w][tid] - s_count[fw][tid] + s_part[q][tid];
+                }
+            }
+        }
+        __syncthreads();
+
+        // Only a tile continuing an array walks back; the first round's words
+        // are requested before the staging, which overlaps their latency.
+        constexpr uint32_t G = BLOCK / RADIX;
+        constexpr uint32_t K = MGPU_LOOKBACK_WINDOW / G > 0 ? MGPU_LOOKBACK_WINDOW / G : 1;
+        const uint32_t lb_digit = tid / G, lb_k = tid % G;
+        uint32_t lb_words[K];
+        if (cont) lookback_read<K, XSEG>(a.lookback, (int32_t)tile - 1, lb_digit, lb_k, lb_words);
+
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            pos[j] += s_count[wave][digit_of(j)];
+            s_stage[pos[j]] = key[j];
+            if constexpr (VMODE == MGPU_VALUES_INDEX) s_vstage[pos[j]] = wbase + lane + j * 64;
+            else if constexpr (VMODE == MGPU_VALUES_SINGLE) s_vstage[pos[j]] = val[j];
+        }
+
+        // Per (part, digit): output position minus stage position.  With one
+        // part the threads that end the walk (or the digits' threads) set it
+        // at once.
+        const uint32_t* const ds0 =
+            a.digit_start + ((size_t)first_arr * MGPU_MAX_PASSES + a.pass) * RADIX;
+        if (cont) {
+            const uint32_t prefix = lookback_window<G, K, XSEG>(
+                a.lookback, tile, lb_digit, lb_k, lane, a.error_flag, a.spin_limit, lb_words);
+            if (lb_k == 0) {
+                if (nparts == 1) {
+                    lookback_publish<XSEG>(&a.lookback[(size_t)tile * RADIX + lb_digit],
+                                           MGPU_LB_PREFIX | (prefix + s_last[lb_digit]));
+                    s_part[0][lb_digit] = ds0[lb_digit] + prefix - s_part[0][lb_digit];
+                } else {
+                    s_prefix[lb_digit] = prefix;
+                }
+            }
+        } else if (tid < RADIX) {
+            if (nparts == 1) s_part[0][tid] = ds0[tid] - s_part[0][tid];
+            else s_prefix[tid] = 0;
+        }
+        __syncthreads();
+        if (nparts > 1) {
+            for (uint32_t pb = 0; pb < nparts; pb += GROUPS) {
+                const uint32_t q = pb + tid / RADIX, d = tid % RADIX;
+                if (q < nparts)
+                    s_part[q][d] = ds0[(size_t)q * MGPU_MAX_PASSES * RADIX + d] +
+                                   (q == 0 ? s_prefix[d] : 0u) - s_part[q][d];
+            }
+            __syncthreads();
+        }
+
+        // Write out: round i covers stage positions [i*BLOCK, (i+1)*BLOCK),
+        // inside chunk i*BLOCK/WT of the stage, hence inside one part; the
+        // rounds go in order and switch arrays where a part starts.
+        // Positions past the part's keys (fillers) get dest = the array's
+        // length, which the buffer stores' range check drops; offsets outside
+        // the array from a broken count or lookback are flagged instead.
+        uint32_t kout[IPT], dest[IPT];
+#pragma unroll
+        for (int i = 0; i < IPT; ++i) kout[i] = s_stage[i * BLOCK + tid];
+        bool bad = false;
+        {
+            uint32_t q = 0;
+            const mgpuSegDesc* od = &a.segs[first_arr];
+            uint32_t on = od->n, pend = s_pend[0];
+            __amdgpu_buffer_rsrc_t rk = array_rsrc(od->keys[out], on);
+            __amdgpu_buffer_rsrc_t rv = JOINT ? array_rsrc(od->vals[0][out], on) : rk;
+#pragma unroll
+            for (int i = 0; i < IPT; ++i) {
+                constexpr uint32_t R = WT / BLOCK;  // rounds per chunk
+                const uint32_t c = (uint32_t)i / R;
+                if ((uint32_t)i % R == 0 && c > 0 && ((starts >> c) & 1u)) {
+                    ++q;
+                    od = &a.segs[first_arr + q];
+                    on = od->n;
+                    pend = s_pend[q];
+                    rk = array_rsrc(od->keys[out], on);
+                    if constexpr (JOINT) rv = array_rsrc(od->vals[0][out], on);
+                }
+                const uint32_t sp = (uint32_t)(i * BLOCK) + tid;
+                const bool real = c < live && sp < pend;
+                const uint32_t dd = s_part[q][__builtin_amdgcn_ubfe(kout[i], shift, width)] + sp;
+                const bool ok = real && dd < on;
+                bad = bad || (real && !ok);
+                dest[i] = ok ? dd : on;
+                __builtin_amdgcn_raw_buffer_store_b32(kout[i], rk, (int)(dest[i] * 4u), 0, 0);
+                if constexpr (JOINT)
+                    __builtin_amdgcn_raw_buffer_store_b32(s_vstage[sp], rv, (int)(dest[i] * 4u), 0, 0);
+            }
+        }
+        if (bad) set_error(a.error_flag, 4u);
+
+        // Two or more value arrays: one after the other through the key stage.
+        if constexpr (VMODE == MGPU_VALUES_MULTI) {
+            for (uint32_t v = 0; v < a.num_values; ++v) {
+                load_striped<IPT>(sd.vals[v][in], n, wbase, lane, val);
+                __syncthreads();
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) s_stage[pos[j]] = val[j];
+                __syncthreads();
+                uint32_t q = 0;
+                const mgpuSegDesc* od = &a.segs[first_arr];
+                __amdgpu_buffer_rsrc_t rv = array_rsrc(od->vals[v][out], od->n);
+#pragma unroll
+                for (int i = 0; i < IPT; ++i) {
+                    constexpr uint32_t R = WT / BLOCK;
+                    const uint32_t c = (uint32_t)i / R;
+                    if ((uint32_t)i % R == 0 && c > 0 && ((starts >> c) & 1u)) {
+                        ++q;
+                        od = &a.segs[first_arr + q];
+                        rv = array_rsrc(od->vals[v][out], od->n);
+                    }
+                    __builtin_amdgcn_raw_buffer_store_b32(s_stage[i * BLOCK + tid], rv,
+                                                          (int)(dest[i] * 4u), 0, 0);
+                }
+            }
+        }
+
+        if constexpr (!XSEG) break;
+        __syncthreads();  // every read of this tile's LDS state is done
+        for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&s_count[0][0])[i] = 0;
+    }  // while
+
+    // The last workgroup to finish checks that every XCD's range was taken
+    // to its end.
+    if constexpr (XSEG) {
+        if (tid == 0) {
+            const uint32_t done =
+                __hip_atomic_fetch_add(a.finished, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (done + 1 == gridDim.x) {
+                for (uint32_t x = 0; x < a.num_xcds; ++x)
+                    if (ld_agent(a.tile_counter + x) < a.xcd_tile_begin[x + 1] - a.xcd_tile_begin[x])
+                        set_error(a.error_flag, 5u);
+            }
+        }
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1289,6 +1667,23 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
     MGPU_RANK_SCATTER_MODE(B, I, R, 2, MGPU_VALUES_SINGLE)                                    \
     MGPU_RANK_SCATTER_MODE(B, I, R, 3, MGPU_VALUES_MULTI)
 
+// Packed device-wide passes of a segmented sort (atomic-add ranking), one
+// chain across the device (_v) or chains per XCD (_x_v).
+#define MGPU_PACKED_MODE(B, I, V, MODE)                                                       \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_multi_packed_b##B##_i##I##_v##V(mgpuMultiPassArgs a) {                           \
+        packed_scatter_body<B, I, MODE, false>(a);                                            \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_multi_packed_b##B##_i##I##_x_v##V(mgpuMultiPassArgs a) {                         \
+        packed_scatter_body<B, I, MODE, true>(a);                                             \
+    }
+#define MGPU_PACKED(B, I)                                                                     \
+    MGPU_PACKED_MODE(B, I, 0, MGPU_VALUES_NONE)                                               \
+    MGPU_PACKED_MODE(B, I, 1, MGPU_VALUES_INDEX)                                              \
+    MGPU_PACKED_MODE(B, I, 2, MGPU_VALUES_SINGLE)                                             \
+    MGPU_PACKED_MODE(B, I, 3, MGPU_VALUES_MULTI)
+
 // Tile shapes: 1024 x 16 (16384 keys, one workgroup per CU: the fastest from
 // 2^21 keys up), 512 x 16 (below that), 256 x 16 (kept for tests and sweeps);
 // each with both ranking methods, with one lookback chain across the device
@@ -1301,6 +1696,9 @@ MGPU_RANK_SCATTER(512, 16, 1)
 MGPU_RANK_SCATTER(1024, 16, 1)
 MGPU_RANK_SCATTER_EE(512, 16)
 MGPU_RANK_SCATTER_EE(1024, 16)
+MGPU_PACKED(256, 16)
+MGPU_PACKED(512, 16)
+MGPU_PACKED(1024, 16)
 
 // Check of the ordering RANK 1 relies on: wave w of one 1024-thread
 // workgroup adds 1 per lane, for 16 rounds, to the count of digit

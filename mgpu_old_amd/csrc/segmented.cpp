@@ -26,6 +26,10 @@ namespace {
 // 0.764 ms; 4096 in [1, 2^16], 2.7 tiles each, 0.673 -> 0.729 ms).
 constexpr uint64_t kXcdMinTiles = 4;
 
+// Longest segment the device-wide count reads with one workgroup, writing
+// its digit starts directly (a 2^18-key segment: 1 MB, a few microseconds).
+constexpr uint32_t kDirectCountMax = 1u << 18;
+
 // Segments of at least wide.minKeys keys go to the device-wide passes (all
 // of them together, each spread over many workgroups with a lookback chain)
 // instead of one workgroup each.  Above 4096 keys (the largest a 256-thread
@@ -48,6 +52,26 @@ uint32_t wide_chunk(const sortEngine_d* e) {
     const char* v = std::getenv("MGPUSORT_SEG_WIDE_CHUNK");
     const long x = v ? std::strtol(v, nullptr, 10) : 0;
     return x >= 1024 ? clamp_chunk(x) : e->segWide.chunk;
+}
+// Tile shape of the device-wide passes: the table's, unless
+// MGPUSORT_SEG_WIDE_TILE=auto|small|large overrides it (measurements).
+int wide_tile(const sortEngine_d* e) {
+    const char* v = std::getenv("MGPUSORT_SEG_WIDE_TILE");
+    if (v && std::strcmp(v, "auto") == 0) return 0;
+    if (v && std::strcmp(v, "small") == 0) return 1;
+    if (v && std::strcmp(v, "large") == 0) return 2;
+    return e->segWide.tile;
+}
+// Packed tiles for the device-wide passes (segments back to back in wave
+// chunks): the table's choice, by default when the segments have one
+// lookback chain each (short chains, where a tile per segment leaves most
+// last tiles part empty); MGPUSORT_SEG_PACK=1 packs always (chains per XCD
+// too), =0 never.
+int pack_mode(const sortEngine_d* e) {
+    const char* v = std::getenv("MGPUSORT_SEG_PACK");
+    if (v && v[0] == '0' && v[1] == 0) return 0;
+    if (v && v[0] == '1' && v[1] == 0) return 1;
+    return e->segWide.pack;
 }
 // Count kernel of the device-wide passes: lane-column histograms in
 // 1024-thread workgroups (default), or per-wave histograms in 256-thread
@@ -114,13 +138,17 @@ sortStatus_t mgpu_seg_load(sortEngine_d* e) {
             if (line.empty() || line[0] == '#') continue;
             std::istringstream is(line);
             if (line.compare(0, 5, "wide ") == 0) {
-                std::string w, tile;
+                std::string w, tile, pack;
                 long minKeys = 0, chunk = 0;
                 if (is >> w >> minKeys >> chunk >> tile && minKeys > 0 && chunk >= 1024 &&
                     (tile == "auto" || tile == "small" || tile == "large")) {
-                    e->segWide.minKeys = (uint32_t)minKeys;
-                    e->segWide.chunk = clamp_chunk(chunk);
-                    e->segWide.tile = tile == "small" ? 1 : tile == "large" ? 2 : 0;
+                    if (!(is >> pack)) pack = "auto";
+                    if (pack == "auto" || pack == "packed" || pack == "unpacked") {
+                        e->segWide.minKeys = (uint32_t)minKeys;
+                        e->segWide.chunk = clamp_chunk(chunk);
+                        e->segWide.tile = tile == "small" ? 1 : tile == "large" ? 2 : 0;
+                        e->segWide.pack = pack == "packed" ? 1 : pack == "unpacked" ? 0 : -1;
+                    }
                 }
                 continue;
             }
@@ -192,10 +220,12 @@ size_t mgpu_seg_payload_bound(uint32_t numSegments, uint64_t keys, uint32_t minT
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t segs = std::max<size_t>(1, numSegments);
     const size_t chunks = keys / 1024 + segs;  // pieces of at least 1024 keys
-    const size_t tiles = keys / std::max(1u, minTile) + segs;
+    // Tiles: one per minTile keys plus a partial one per segment (packed:
+    // plus padding to a tile boundary per XCD), two words each when packed.
+    const size_t tiles = keys / std::max(1u, minTile) + segs + MGPU_MAX_SEGS;
     return 2 * al(sizeof(mgpuSegDesc) * segs) + al(sizeof(uint32_t) * segs) +
            al(sizeof(uint32_t) * (segs + 1)) + al(4 * sizeof(uint32_t) * chunks) +
-           al(sizeof(uint32_t) * tiles) + al(sizeof(uint32_t) * (MGPU_MAX_SEGS + 1));
+           al(2 * sizeof(uint32_t) * tiles) + al(sizeof(uint32_t) * (MGPU_MAX_SEGS + 1));
 }
 
 sortStatus_t mgpu_arena_reserve(SegArena* arena, size_t payload, size_t counterWords,
@@ -288,25 +318,35 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         P.launches.push_back(L);
     }
 
-    // Device-wide segments: their descriptors, first tile of each, and the
+    // Device-wide segments: their descriptors, their tile layout, and the
     // digit_count chunks.
     std::vector<mgpuSegDesc> wideDescs;
     std::vector<uint32_t> tileBegin, chunks, tileSeg;
+    P.widePacked = false;
     if (!P.wide.empty()) {
         uint64_t wideKeys = 0;
         for (uint32_t k : P.wide) wideKeys += segs[k].n;
-        // Each segment's last tile is partial, so segments of a few tiles
+        // Packed tiles leave no partial tile per segment, so they take the
+        // shape sortArray would take for all of the keys.  Without packing,
+        // each segment's last tile is partial, so segments of a few tiles
         // take the small tile shape (4096 segments log-uniform in [1, 2^16]:
         // 0.74 -> 0.69 ms); the large shape once they average 4 large tiles.
+        const int packMode = pack_mode(e);
+        const bool packBuilt = e->large.packed[0] != nullptr && e->small.packed[0] != nullptr;
         const uint64_t avgKeys = wideKeys / P.wide.size();
-        if (e->segWide.tile == 1) P.wideVariant = &e->small;
-        else if (e->segWide.tile == 2) P.wideVariant = &e->large;
+        const int tileChoice = wide_tile(e);
+        if (tileChoice == 1) P.wideVariant = &e->small;
+        else if (tileChoice == 2) P.wideVariant = &e->large;
+        else if (packBuilt && packMode != 0) P.wideVariant = &mgpu_pick_variant(e, wideKeys);
         else P.wideVariant = avgKeys >= 4ull * e->large.tile() ? &mgpu_pick_variant(e, wideKeys) : &e->small;
         const uint32_t tile = P.wideVariant->tile();
+        const uint32_t waves = P.wideVariant->blockThreads / 64;
+        const uint32_t chunkLen = 64 * P.wideVariant->itemsPerThread;  // packed: keys per wave chunk
+        const uint32_t unit = tile;  // work of a segment for the XCD choice: its tiles
         // Lookback chains per XCD: each segment's tiles on one XCD, whose L2
         // alone holds its chain (~0.45 us a round trip instead of ~0.85 us
         // through the device-coherent level).  Segments go to the XCD with
-        // the fewest tiles so far, largest first, and are then laid out XCD
+        // the least work so far, largest first, and are then laid out XCD
         // by XCD.  Only for segments of longer chains (at least kXcdMinTiles
         // tiles on average: short chains wait little, and the workgroups
         // that loop over an XCD's tiles pay a counter round trip and a
@@ -315,11 +355,14 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         // MGPUSORT_SEG_XCD=0|1 forces the choice.
         P.wideXcds = 1;
         P.xcdTileBegin.clear();
+        std::vector<size_t> groupEnd(1, P.wide.size());  // end of each XCD's segments
         const uint32_t X = e->xcdSegs;
-        if (X > 1 && P.wideVariant->multiX[MGPU_VALUES_NONE]) {
+        const bool xcdKernels = P.wideVariant->multiX[MGPU_VALUES_NONE] != nullptr &&
+                                (packMode != 1 || P.wideVariant->packedX[MGPU_VALUES_NONE] != nullptr);
+        if (X > 1 && xcdKernels) {
             uint64_t total = 0, most = 0;
             for (uint32_t k : P.wide) {
-                const uint64_t t = (segs[k].n + tile - 1) / tile;
+                const uint64_t t = (segs[k].n + unit - 1) / unit;
                 total += t;
                 most = std::max(most, t);
             }
@@ -337,31 +380,94 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
                     const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) -
                                                   load.begin());
                     group[x].push_back(k);
-                    load[x] += (segs[k].n + tile - 1) / tile;
+                    load[x] += (segs[k].n + unit - 1) / unit;
                 }
                 P.wide.clear();
+                groupEnd.clear();
                 P.xcdTileBegin.push_back(0);
                 for (uint32_t x = 0; x < X; ++x) {
                     P.wide.insert(P.wide.end(), group[x].begin(), group[x].end());
+                    groupEnd.push_back(P.wide.size());
                     P.xcdTileBegin.push_back(P.xcdTileBegin.back() + (uint32_t)load[x]);
                 }
                 P.wideXcds = X;
             }
         }
-        tileBegin.push_back(0);
+        P.widePacked = P.wideVariant->packed[MGPU_VALUES_NONE] != nullptr &&
+                       (packMode == 1 || (packMode == -1 && P.wideXcds == 1));
+        // The count reads each segment as one piece and writes its digit
+        // starts itself when no segment is longer than kDirectCountMax keys
+        // (one workgroup per segment stays short) and no piece size was
+        // asked for; else pieces of chunkKeys with a histogram in global
+        // memory and a scan launch after them.
+        uint32_t longest = 0;
+        for (uint32_t k : P.wide) longest = std::max(longest, segs[k].n);
+        P.wideDirectCount = count_cols() && longest <= kDirectCountMax &&
+                            std::getenv("MGPUSORT_SEG_WIDE_CHUNK") == nullptr;
+        const uint32_t pieceKeys = P.wideDirectCount ? longest : chunkKeys;
         for (size_t k = 0; k < P.wide.size(); ++k) {
             const mgpuSegDesc& d = segs[P.wide[k]];
             wideDescs.push_back(d);
-            tileBegin.push_back(tileBegin.back() + (d.n + tile - 1) / tile);
-            tileSeg.resize(tileBegin.back(), (uint32_t)k);
-            for (uint32_t b = 0; b < d.n; b += chunkKeys) {
+            for (uint32_t b = 0; b < d.n; b += pieceKeys) {
                 chunks.push_back((uint32_t)k);
                 chunks.push_back(b);
-                chunks.push_back(std::min(d.n, b + chunkKeys));
+                chunks.push_back(std::min(d.n, b + pieceKeys));
                 chunks.push_back(0);
             }
         }
-        P.wideTiles = tileBegin.back();
+        if (P.widePacked) {
+            // Wave chunks back to back, `waves` to a tile.  A segment that
+            // would be the (max parts + 1)-th part of a tile starts the next
+            // tile instead, and each XCD's segments end on a tile boundary:
+            // padding chunks only ever end a tile.
+            const uint32_t maxParts = MGPU_PACK_MAX_PARTS(waves);
+            std::vector<uint64_t> begin(P.wide.size());
+            auto chunksOf = [&](size_t k) { return ((uint64_t)segs[P.wide[k]].n + chunkLen - 1) / chunkLen; };
+            uint64_t cpos = 0;
+            uint32_t parts = 0;  // parts of the tile holding chunk cpos
+            std::vector<uint32_t> xb(1, 0);
+            size_t k = 0;
+            for (size_t g = 0; g < groupEnd.size(); ++g) {
+                for (; k < groupEnd[g]; ++k) {
+                    if (cpos % waves != 0 && parts >= maxParts) cpos += waves - cpos % waves;
+                    if (cpos % waves == 0) parts = 0;
+                    begin[k] = cpos;
+                    const uint64_t end = cpos + chunksOf(k);
+                    if (end / waves == cpos / waves) ++parts;
+                    else parts = end % waves != 0 ? 1u : 0u;
+                    cpos = end;
+                }
+                if (cpos % waves != 0) cpos += waves - cpos % waves;
+                parts = 0;
+                xb.push_back((uint32_t)(cpos / waves));
+            }
+            const uint64_t tiles = cpos / waves;
+            if (cpos >= (1ull << 32)) return SORT_STATUS_INVALID_VALUE;
+            tileBegin.resize(P.wide.size());
+            for (size_t i = 0; i < P.wide.size(); ++i) tileBegin[i] = (uint32_t)begin[i];
+            tileSeg.assign(2 * tiles, 0);
+            size_t j = 0;
+            for (uint64_t t = 0; t < tiles; ++t) {
+                const uint64_t c0 = t * waves;
+                while (j + 1 < P.wide.size() && begin[j + 1] <= c0) ++j;
+                uint32_t startMask = 0, live = 0;
+                for (size_t i = j; i < P.wide.size() && begin[i] < c0 + waves; ++i) {
+                    if (begin[i] > c0) startMask |= 1u << (uint32_t)(begin[i] - c0);
+                    live = (uint32_t)std::min<uint64_t>(waves, begin[i] + chunksOf(i) - c0);
+                }
+                tileSeg[2 * t] = (uint32_t)j;
+                tileSeg[2 * t + 1] = startMask | (live << 16) | ((begin[j] < c0 ? 1u : 0u) << 24);
+            }
+            P.wideTiles = (uint32_t)tiles;
+            if (P.wideXcds > 1) P.xcdTileBegin = xb;
+        } else {
+            tileBegin.push_back(0);
+            for (size_t k = 0; k < P.wide.size(); ++k) {
+                tileBegin.push_back(tileBegin.back() + (segs[P.wide[k]].n + tile - 1) / tile);
+                tileSeg.resize(tileBegin.back(), (uint32_t)k);
+            }
+            P.wideTiles = tileBegin.back();
+        }
         P.wideChunks = (uint32_t)(chunks.size() / 4);
     }
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
@@ -513,18 +619,23 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         (uint64_t)e->numCUs * (cols ? 2 : 8));
     grid = std::max(grid, clearGrid);
     if (!P.ownMulti) lane.multiDirty = true;
+    ca.digit_start = digitStart;
+    ca.tile_counters = lane.tileCounters;
+    ca.direct = P.wideDirectCount ? 1u : 0u;
     sortStatus_t st = mgpu_launch(e, lane.stream, cols ? e->fnMultiCountCols : e->fnMultiCount,
                                   grid, threads, &ca);
     if (st) return st;
 
-    mgpuMultiScanArgs sa;
-    sa.hist = hist;
-    sa.digit_start = digitStart;
-    sa.tile_counters = lane.tileCounters;
-    sa.num_passes = np;
-    sa.pad_ = 0;
-    st = mgpu_launch(e, lane.stream, e->fnMultiScan, numWide, 256, &sa);
-    if (st) return st;
+    if (!P.wideDirectCount) {
+        mgpuMultiScanArgs sa;
+        sa.hist = hist;
+        sa.digit_start = digitStart;
+        sa.tile_counters = lane.tileCounters;
+        sa.num_passes = np;
+        sa.pad_ = 0;
+        st = mgpu_launch(e, lane.stream, e->fnMultiScan, numWide, 256, &sa);
+        if (st) return st;
+    }
     if (!P.ownMulti) lane.multiDirty = false;
 
     const int carried = P.valueCount == -1 ? 1 : P.valueCount;
@@ -551,18 +662,27 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
         else if (P.valueCount == -1) mode = p == 0 ? MGPU_VALUES_INDEX : MGPU_VALUES_SINGLE;
         else if (P.valueCount == 1) mode = MGPU_VALUES_SINGLE;
         else mode = MGPU_VALUES_MULTI;
+        if (P.widePacked) {
+            pa.chunk_begin = P.devTileBegin;
+            pa.tile_info = P.devTileSeg;
+            pa.tile_begin = nullptr;
+            pa.tile_seg = nullptr;
+        }
         if (P.wideXcds > 1) {
             // Workgroups that loop over their XCD's tiles: as many as fit.
             pa.xcd_tile_begin = P.devXcdTileBegin;
             pa.num_xcds = P.wideXcds;
             pa.finished = lane.tileCounters + MGPU_FINISHED_SLOT + p;
-            const uint32_t g =
-                std::max(P.wideXcds, std::min(P.wideTiles, P.wideVariant->multiXGrid));
-            st = mgpu_launch(e, lane.stream, P.wideVariant->multiX[mode], g,
-                             P.wideVariant->blockThreads, &pa);
+            const uint32_t fit =
+                P.widePacked ? P.wideVariant->packedXGrid : P.wideVariant->multiXGrid;
+            const uint32_t g = std::max(P.wideXcds, std::min(P.wideTiles, fit));
+            st = mgpu_launch(e, lane.stream,
+                             P.widePacked ? P.wideVariant->packedX[mode] : P.wideVariant->multiX[mode],
+                             g, P.wideVariant->blockThreads, &pa);
         } else {
-            st = mgpu_launch(e, lane.stream, P.wideVariant->multi[mode], P.wideTiles,
-                             P.wideVariant->blockThreads, &pa);
+            st = mgpu_launch(e, lane.stream,
+                             P.widePacked ? P.wideVariant->packed[mode] : P.wideVariant->multi[mode],
+                             P.wideTiles, P.wideVariant->blockThreads, &pa);
         }
         if (st) return st;
     }

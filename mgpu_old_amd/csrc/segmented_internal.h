@@ -44,11 +44,13 @@ struct SegTuneEntry {
 
 // The device-wide passes of a segmented sort: segments of at least minKeys
 // keys take them; digit_count reads pieces of at most chunk keys; tile shape
-// (0: small unless segments average four large tiles, 1 small, 2 large).
+// (0: auto, 1 small, 2 large); packed tiles (-1: when the segments have one
+// chain each, 0 never, 1 always).
 struct SegWideTune {
     uint32_t minKeys = 4096 + 1;
     uint32_t chunk = 1u << 14;
     int tile = 0;
+    int pack = -1;
 };
 
 // One persistent launch of one kernel variant.
@@ -89,9 +91,17 @@ struct SegPlanData {
     const PassVariant* wideVariant = nullptr;  // tile shape of the device-wide passes
     uint32_t wideTiles = 0;      // tiles of all device-wide segments
     uint32_t wideChunks = 0;     // digit_count chunks of them
+    // Packed tiles (mgpu_multi_packed_*): the segments back to back in wave
+    // chunks; devTileBegin then holds each segment's first chunk and
+    // devTileSeg two words per tile (its first segment; part starts, chunks
+    // holding keys and whether it continues a segment).
+    bool widePacked = false;
+    // The count kernel reads every device-wide segment as one piece and
+    // writes its digit starts (no global histogram, no scan launch).
+    bool wideDirectCount = false;
     mgpuSegDesc* devWide = nullptr;
-    uint32_t* devTileBegin = nullptr;
-    uint32_t* devTileSeg = nullptr;  // [wideTiles]: the device-wide segment of each tile
+    uint32_t* devTileBegin = nullptr;  // [segments + 1]: first tile of each (packed: first chunk)
+    uint32_t* devTileSeg = nullptr;    // [wideTiles]: the segment of each tile (packed: tile words)
     uint32_t* devChunks = nullptr;
     // Lookback chains per XCD (wideXcds > 1): the device-wide segments are
     // grouped by XCD, XCD x holding global tiles [xcdTileBegin[x],

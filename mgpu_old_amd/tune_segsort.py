@@ -7,8 +7,10 @@ creation (kernelPath + "mgpusort_seg_tuning.txt"):
      with every segment kept in the persistent kernel;
   2. the device-wide passes, on the batch they serve (4096 segments with
      sizes log-uniform in [1, 2^16], key + one value): the size from which a
-     segment takes them, the keys per count piece and the tile shape, timed
-     with the bucket rows of step 1 in force.
+     segment takes them, the tile shape and whether tiles are packed
+     (segments back to back) or one segment's own, timed with the bucket
+     rows of step 1 in force (the count piece size only matters for segments
+     above 2^18 keys, which this batch does not have: it stays at 2^14).
 
 It plays the role of the reference's offline pass-table generator
 (sort/src/tablegen/tablegen.cpp:9-64 over measured rates in throughputs.txt),
@@ -31,9 +33,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUCKETS = [(0, 6), (6, 8), (8, 10), (10, 12), (12, 13)]
 VARIANTS = [f"r{rb}i{ipt}b{b}" for rb in range(4, 9) for ipt in (8, 16) for b in (128, 256, 512)
             if b >= (1 << rb)]
-WIDE_MIN = [1025, 2049, 4097, 8193]
-WIDE_CHUNK = [1 << 14, 1 << 16]
+WIDE_MIN = [1025, 2049, 4097]
+WIDE_CHUNK = [1 << 14]
 WIDE_TILE = ["small", "large"]
+WIDE_PACK = ["packed", "unpacked"]
 
 
 def bucket_sizes(lo, hi, total, seed):
@@ -54,7 +57,8 @@ def bucket_sizes(lo, hi, total, seed):
 
 def time_plan(torch, engine, keys, vals, starts, reps):
     """Mean time of `reps` replays of a captured segmented plan (key + one
-    value, bits [0, 32)), after one warm-up replay."""
+    value, bits [0, 32)), after one warm-up replay: the median of three
+    such runs."""
     import mgpu_old_amd as mg
     n = keys.numel()
     k1, v1 = torch.empty_like(keys), torch.empty_like(vals)
@@ -67,14 +71,16 @@ def time_plan(torch, engine, keys, vals, starts, reps):
     plan = engine.segmented_plan(d, starts)
     plan.execute(d)  # warm-up (32-bit keys: 4 passes, the buffers keep their orientation)
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        plan.execute(d)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / reps
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.execute(d)
+        torch.cuda.synchronize()
+        runs.append((time.perf_counter() - t0) / reps)
     plan.close()
     engine.check_device()
-    return dt
+    return sorted(runs)[1]
 
 
 def engine_on(torch, dirpath, table_text):
@@ -142,12 +148,13 @@ def main():
         for m in WIDE_MIN:
             for c in WIDE_CHUNK:
                 for t in WIDE_TILE:
-                    line = f"wide {m} {c} {t}"
-                    eng = engine_on(torch, tmp, bucket_text + line + "\n")
-                    wide[line] = time_plan(torch, eng, keys, vals, starts, args.reps)
-                    eng.close()
-                    print(f"[tune] 4096 segments ({n} keys) {line}: {wide[line] * 1e3:.3f} ms",
-                          flush=True)
+                    for pk in WIDE_PACK:
+                        line = f"wide {m} {c} {t} {pk}"
+                        eng = engine_on(torch, tmp, bucket_text + line + "\n")
+                        wide[line] = time_plan(torch, eng, keys, vals, starts, args.reps)
+                        eng.close()
+                        print(f"[tune] 4096 segments ({n} keys) {line}: {wide[line] * 1e3:.3f} ms",
+                              flush=True)
         best_wide = min(wide, key=wide.get)
         report["wide"] = {"best": best_wide, "keys": n, "ms": {k: t * 1e3 for k, t in wide.items()}}
     finally:
@@ -156,7 +163,8 @@ def main():
     with open(args.out, "w") as f:
         f.write("# segmented-sort tuning table, written by mgpu_old_amd/tune_segsort.py (key + 1 value,\n"
                 "# bits [0,32)): <max ceil(log2 n)> <radix bits> <keys per thread> <threads> per\n"
-                "# size bucket of the persistent kernel; wide <min keys> <count piece> <tile>\n")
+                "# size bucket of the persistent kernel; wide <min keys> <count piece> <tile>\n"
+                "# <packed|unpacked>\n")
         f.write(bucket_text)
         f.write(best_wide + "\n")
     print(json.dumps({"table": rows, "wide": best_wide, "report": report}), flush=True)

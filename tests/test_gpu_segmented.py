@@ -215,7 +215,7 @@ def test_index_mode_needs_both_value_sides(engine, eb):
 
 
 @pytest.mark.parametrize("layout", ["xcd", "one_chain", "small_pieces", "wave_count",
-                                    "dominant_segment"])
+                                    "dominant_segment", "unpacked", "packed_xcd"])
 def test_device_wide_layouts(oracle, layout):
     """The device-wide passes of a segmented sort in each layout: lookback
     chains per XCD with whole segments grouped by XCD (MGPUSORT_SEG_XCD=1),
@@ -223,15 +223,21 @@ def test_device_wide_layouts(oracle, layout):
     holding more than a quarter of an XCD's share of the tiles, even when
     asked for chains per XCD), count pieces smaller than a segment
     (MGPUSORT_SEG_WIDE_CHUNK=4096, one chain per segment:
-    MGPUSORT_SEG_XCD=0), and the per-wave count kernel instead of the
-    lane-column one (MGPUSORT_SEG_COUNT=wave).  Every segment bit-exact
-    against the oracle."""
+    MGPUSORT_SEG_XCD=0; a histogram in global memory and a scan launch
+    after the count), the per-wave count kernel instead of the lane-column
+    one (MGPUSORT_SEG_COUNT=wave).  Tiles are packed (segments back to
+    back) by default with one chain per segment, and a tile per segment
+    with chains per XCD; MGPUSORT_SEG_PACK=0 gives one chain per segment
+    tiles of their own, MGPUSORT_SEG_PACK=1 packs chains per XCD too.
+    Every segment bit-exact against the oracle."""
     import torch
     env = {"MGPUSORT_XCD": "0"} if layout == "one_chain" else {}
     eng = make_engine(env)
     knobs = {"xcd": {"MGPUSORT_SEG_XCD": "1"}, "dominant_segment": {"MGPUSORT_SEG_XCD": "1"},
              "small_pieces": {"MGPUSORT_SEG_WIDE_CHUNK": "4096", "MGPUSORT_SEG_XCD": "0"},
-             "wave_count": {"MGPUSORT_SEG_COUNT": "wave"}}.get(layout, {})
+             "wave_count": {"MGPUSORT_SEG_COUNT": "wave"},
+             "unpacked": {"MGPUSORT_SEG_PACK": "0", "MGPUSORT_SEG_XCD": "0"},
+             "packed_xcd": {"MGPUSORT_SEG_PACK": "1", "MGPUSORT_SEG_XCD": "1"}}.get(layout, {})
     saved = {k: os.environ.get(k) for k in knobs}
     try:
         os.environ.update(knobs)
@@ -250,12 +256,25 @@ def test_device_wide_layouts(oracle, layout):
             names = set(eng.read_timing())
             eng.enable_timing(False)
             _check_segments(oracle, keys, vv, vc, starts, g.keys(), g.values(), fb, eb)
-            per_xcd = any(k.startswith("mgpu_multi_scatter") and "_x_" in k for k in names)
-            if layout == "xcd" and eng.info()["xcdSegments"] > 1:
+            scatter = {k for k in names if k.startswith("mgpu_multi_")
+                       and not k.startswith("mgpu_multi_count") and k != "mgpu_multi_scan"}
+            assert scatter, names
+            per_xcd = any("_x_" in k for k in scatter)
+            xcds = eng.info()["xcdSegments"] > 1
+            packed = {"one_chain": True, "small_pieces": True, "dominant_segment": True,
+                      "unpacked": False, "packed_xcd": True, "xcd": not xcds}.get(layout)
+            if packed is not None:
+                assert all(k.startswith("mgpu_multi_packed" if packed else "mgpu_multi_scatter")
+                           for k in scatter), names
+            if layout in ("xcd", "packed_xcd") and xcds:
                 assert per_xcd, names
-            if layout in ("one_chain", "dominant_segment", "small_pieces"):
+            if layout in ("one_chain", "dominant_segment", "small_pieces", "unpacked"):
                 assert not per_xcd, names
             assert ("mgpu_multi_count_cols" in names) == (layout != "wave_count"), names
+            # Segments of at most 2^18 keys, no piece size asked for: the count
+            # writes the digit starts itself.
+            assert ("mgpu_multi_scan" in names) == (
+                layout in ("small_pieces", "wave_count", "dominant_segment")), names
     finally:
         for k, v in saved.items():
             if v is None:
@@ -296,7 +315,60 @@ def test_seg_table_dimensions_honoured(tmp_path, oracle):
         seg_kernels = {k for k in names if k.startswith("mgpu_segsort")}
         assert seg_kernels == {"mgpu_segsort_r5_i8_b128", "mgpu_segsort_r8_i16_b512",
                                "mgpu_segsort_r7_i16_b256"}, names
-        scatter = {k for k in names if k.startswith("mgpu_multi_scatter")}
+        scatter = {k for k in names if k.startswith("mgpu_multi_packed")}
         assert scatter and all("_b1024_" in k for k in scatter), names
     finally:
+        eng.close()
+
+
+@pytest.mark.parametrize("tile", ["small", "large"])
+@pytest.mark.parametrize("xcd", ["0", "1"])
+def test_packed_tiles_many_parts(tmp_path, oracle, tile, xcd):
+    """Packed device-wide tiles holding many segments: with every segment
+    routed device-wide and packed by a table row `wide 1 16384 <tile>
+    packed`, segments of 1 to ~3000 keys fill tiles with as many parts as a
+    tile may hold, so the layout pads tiles, starts parts mid-tile and ends
+    segments on partial chunks; chains per XCD too (MGPUSORT_SEG_XCD=1).
+    Keys only, index generation, one and six value arrays, over several bit
+    ranges; every segment bit-exact against the oracle."""
+    import shutil
+    import torch
+    for name in ("mgpusort_radix.hsaco", "mgpusort_seg.hsaco"):
+        shutil.copy(os.path.join(mg.LIB_DIR, name), tmp_path / name)
+    (tmp_path / "mgpusort_seg_tuning.txt").write_text(f"wide 1 16384 {tile} packed\n")
+    saved = {k: os.environ.get(k) for k in ("MGPUSORT_SEG_XCD", "MGPUSORT_SEG_PACK")}
+    os.environ["MGPUSORT_SEG_XCD"] = xcd
+    os.environ.pop("MGPUSORT_SEG_PACK", None)
+    eng = mg.Engine(str(tmp_path) + "/")
+    eng.set_stream(torch.cuda.current_stream().cuda_stream)
+    try:
+        rng = np.random.default_rng(340 + int(xcd) + (tile == "large") * 2)
+        sizes = [int(x) for x in rng.choice([1, 2, 5, 63, 64, 65, 1023, 1024, 1025, 2047, 2049, 3000],
+                                            size=400)]
+        sizes[10:14] = [0, 0, 9_000, 0]
+        starts = _segments(sizes)
+        n = int(starts[-1])
+        keys = workloads.uniform_keys(n, seed=341)
+        vals = [workloads.uniform_keys(n, seed=342 + j) for j in range(6)]
+        for vc, fb, eb in [(0, 0, 32), (-1, 4, 20), (1, 0, 32), (6, 8, 32), (1, 0, 7)]:
+            vv = [] if vc in (0, -1) else vals[:vc]
+            eng.enable_timing(True)
+            eng.read_timing()
+            g = _run_segmented(eng, keys, vv, vc, starts, fb, eb)
+            names = set(eng.read_timing())
+            eng.enable_timing(False)
+            _check_segments(oracle, keys, vv, vc, starts, g.keys(), g.values(), fb, eb)
+            packed = {k for k in names if k.startswith("mgpu_multi_packed")}
+            assert packed, names
+            assert not any(k.startswith("mgpu_segsort") for k in names), names
+            if xcd == "1" and eng.info()["xcdSegments"] > 1:
+                assert all("_x_" in k for k in packed), names
+            if xcd == "0":
+                assert not any("_x_" in k for k in packed), names
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
         eng.close()
