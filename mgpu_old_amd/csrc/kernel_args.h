@@ -171,8 +171,8 @@ struct mgpuSegArgs {
     const mgpuSegDesc* segs;
     const uint32_t* big_list;    // segments for the workgroup routines, largest first
     const uint32_t* small_list;  // segments for the one-wave routine
-    uint32_t* counters;          // [0] big cursor, [1] small cursor, [2] finished
-                                 // workgroups: zero on entry, left zero
+    uint32_t* counters;          // [0] big cursor, [2] finished workgroups ([1],
+                                 // [3] unused): zero on entry, left zero
     uint32_t* error_flag;
     uint32_t num_big;
     uint32_t num_small;
@@ -182,6 +182,9 @@ struct mgpuSegArgs {
     uint32_t num_passes;   // <= MGPU_SEG_MAX_PASSES
     uint32_t shift[MGPU_SEG_MAX_PASSES];
     uint32_t mask[MGPU_SEG_MAX_PASSES];
+    // 1: rank by LDS atomic adds, which the engine checked are served in
+    // ascending lane order; 0: by match masks.
+    uint32_t rank1;
 };
 
 // ---------------------------------------------------------------------------

@@ -13,10 +13,10 @@
 //   global (larger)          one workgroup runs every pass through global
 //                            memory, tile by tile, with running digit offsets
 //                            (no cross-workgroup lookback needed).
-// Workgroups first drain the block/global list (largest first), then their
-// waves drain the small list one segment per wave.  The last workgroup to
-// finish puts the queue cursors back to zero, so a captured graph of this
-// launch can be replayed as is.
+// Workgroups first drain the block/global list (largest first, from a
+// shared cursor), then their waves take the small list round-robin, one
+// segment per wave at a time.  The last workgroup to finish puts the cursor
+// back to zero, so a captured graph of this launch can be replayed as is.
 //
 // In-LDS routines carry (key, source index) through the passes and gather
 // every value array once at the end; the global routine carries the values
@@ -40,6 +40,28 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Stable rank of this lane's digit d among the active lanes of the wave
+// with the same digit, counted from count[d], which it advances.  rank1: one
+// LDS atomic add per lane (served in ascending lane order, as the engine
+// checked at creation), or a single add when every active lane holds the
+// same digit (sorted or constant keys would serialize 64 adds on one word);
+// else by match masks (wave_rank).
+__device__ __forceinline__ uint32_t seg_rank(uint32_t d, uint32_t lane, uint32_t* count,
+                                             uint64_t* match, bool rank1) {
+    if (!rank1) return wave_rank(d, lane, count, match);
+    const uint32_t f = __builtin_amdgcn_readfirstlane(d);
+    const uint64_t exec = __builtin_amdgcn_read_exec();
+    if (__builtin_amdgcn_ballot_w64(d == f) == exec) {
+        const uint32_t below = mgpu::lanes_below(exec);
+        uint32_t b = 0;
+        if (below == 0)
+            b = __hip_atomic_fetch_add(&count[f], (uint32_t)__builtin_popcountll(exec),
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        return __builtin_amdgcn_readfirstlane(b) + below;
+    }
+    return __hip_atomic_fetch_add(&count[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Shared memory of one workgroup of BLOCK threads (W waves).
@@ -121,7 +143,8 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
         uint32_t rank[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) rank[j] = wave_rank((key[j] >> shift) & mask, lane, count, match);
+            if (idx[j] < n)
+                rank[j] = seg_rank((key[j] >> shift) & mask, lane, count, match, a.rank1 != 0);
         wave_sync();
         // Exclusive scan of the digit counts across the wave.
         uint32_t c[DPL], sum = 0;
@@ -187,7 +210,8 @@ __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
             if (idx[j] < n)
-                rank[j] = wave_rank((key[j] >> shift) & mask, lane, sh.count[wave], sh.match[wave]);
+                rank[j] = seg_rank((key[j] >> shift) & mask, lane, sh.count[wave], sh.match[wave],
+                                   a.rank1 != 0);
         __syncthreads();
         uint32_t total = 0;
         if (tid < (uint32_t)RADIX) {
@@ -270,8 +294,8 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
                 const uint32_t i = wb + j * 64 + lane;
                 if (i < n) {
                     key[j] = ki[i];
-                    rank[j] = wave_rank((key[j] >> shift) & mask, lane, sh.count[wave],
-                                        sh.match[wave]);
+                    rank[j] = seg_rank((key[j] >> shift) & mask, lane, sh.count[wave],
+                                       sh.match[wave], a.rank1 != 0);
                 }
             }
             __syncthreads();
@@ -378,16 +402,12 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
         __syncthreads();
     }
 
-    // Wave phase: one small segment per wave at a time.
-    while (true) {
-        uint32_t item = 0;
-        if (lane == 0)
-            item = __hip_atomic_fetch_add(&a.counters[1], 1u, __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-        item = __shfl(item, 0, 64);
-        if (item >= a.num_small) break;
+    // Wave phase: one small segment per wave at a time, dealt round-robin
+    // over the grid's waves.  (A shared cursor costs a device-wide atomic on
+    // one word per segment: ~25 ns each, serialized, which bounded a batch
+    // of 130K segments of up to 64 keys at 3.3 ms whatever the kernel.)
+    for (uint32_t item = blockIdx.x * W + wave; item < a.num_small; item += gridDim.x * W)
         wave_sort<RB, IPT, BLOCK>(a, a.segs[a.small_list[item]], sh, lane, wave);
-    }
 
     __syncthreads();
     if (tid == 0) {
@@ -396,7 +416,6 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
                                                      __HIP_MEMORY_SCOPE_AGENT);
         if (done == gridDim.x - 1) {  // every workgroup is past both queues
             st_agent(&a.counters[0], 0u);
-            st_agent(&a.counters[1], 0u);
             st_agent(&a.counters[2], 0u);
         }
     }

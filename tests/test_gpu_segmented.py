@@ -225,16 +225,18 @@ def test_device_wide_layouts(oracle, layout):
     (MGPUSORT_SEG_WIDE_CHUNK=4096, one chain per segment:
     MGPUSORT_SEG_XCD=0; a histogram in global memory and a scan launch
     after the count), the per-wave count kernel instead of the lane-column
-    one (MGPUSORT_SEG_COUNT=wave).  Tiles are packed (segments back to
-    back) by default with one chain per segment, and a tile per segment
-    with chains per XCD; MGPUSORT_SEG_PACK=0 gives one chain per segment
-    tiles of their own, MGPUSORT_SEG_PACK=1 packs chains per XCD too.
-    Every segment bit-exact against the oracle."""
+    one (MGPUSORT_SEG_COUNT=wave).  Packed tiles (segments back to back,
+    MGPUSORT_SEG_PACK=1) with one chain per segment or chains per XCD, and
+    tiles of each segment's own (MGPUSORT_SEG_PACK=0).  Every segment
+    bit-exact against the oracle."""
     import torch
     env = {"MGPUSORT_XCD": "0"} if layout == "one_chain" else {}
     eng = make_engine(env)
-    knobs = {"xcd": {"MGPUSORT_SEG_XCD": "1"}, "dominant_segment": {"MGPUSORT_SEG_XCD": "1"},
-             "small_pieces": {"MGPUSORT_SEG_WIDE_CHUNK": "4096", "MGPUSORT_SEG_XCD": "0"},
+    knobs = {"xcd": {"MGPUSORT_SEG_XCD": "1"},
+             "one_chain": {"MGPUSORT_SEG_PACK": "1"},
+             "dominant_segment": {"MGPUSORT_SEG_XCD": "1", "MGPUSORT_SEG_PACK": "1"},
+             "small_pieces": {"MGPUSORT_SEG_WIDE_CHUNK": "4096", "MGPUSORT_SEG_XCD": "0",
+                              "MGPUSORT_SEG_PACK": "1"},
              "wave_count": {"MGPUSORT_SEG_COUNT": "wave"},
              "unpacked": {"MGPUSORT_SEG_PACK": "0", "MGPUSORT_SEG_XCD": "0"},
              "packed_xcd": {"MGPUSORT_SEG_PACK": "1", "MGPUSORT_SEG_XCD": "1"}}.get(layout, {})
@@ -262,7 +264,7 @@ def test_device_wide_layouts(oracle, layout):
             per_xcd = any("_x_" in k for k in scatter)
             xcds = eng.info()["xcdSegments"] > 1
             packed = {"one_chain": True, "small_pieces": True, "dominant_segment": True,
-                      "unpacked": False, "packed_xcd": True, "xcd": not xcds}.get(layout)
+                      "unpacked": False, "packed_xcd": True}.get(layout)
             if packed is not None:
                 assert all(k.startswith("mgpu_multi_packed" if packed else "mgpu_multi_scatter")
                            for k in scatter), names
