@@ -242,7 +242,10 @@ struct mgpuMultiPassArgs {
     // Chains per XCD (kernels built for it only): the arrays are grouped by
     // XCD, XCD x holding global tiles [xcd_tile_begin[x], xcd_tile_begin[x+1])
     // of whole arrays, whose lookback chains its workgroups alone read and
-    // write; finished counts the workgroups done (zero on entry).
+    // write; finished counts the workgroups done (zero on entry).  The
+    // kernels of one tile per workgroup read the same arrays, with
+    // num_xcds > 1, as groups of arrays whose tiles come from a counter of
+    // their own (tile_counter[g]).
     const uint32_t* xcd_tile_begin;  // [num_xcds + 1]
     uint32_t* finished;
     uint32_t num_xcds;

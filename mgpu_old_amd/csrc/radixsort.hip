@@ -771,6 +771,30 @@ __device__ __forceinline__ uint32_t* job_vals_out(const Args& a, const TileJob& 
     else return a.vals_out[v];
 }
 
+// One tile per workgroup over several arrays whose tiles come in
+// a.num_xcds groups (whole arrays, group g holding global tiles
+// [xcd_tile_begin[g], xcd_tile_begin[g + 1])), each group with its own
+// counter: workgroup b asks group b % groups first, then the groups after
+// it.  One counter for every tile of a launch serializes the workgroups'
+// atomics on one word (~11 ns each: 3000 tiles, 38 us); spread over 8 words
+// they overlap.  A tile only waits on earlier tiles of its own array, which
+// its group's counter handed out before it, so no tile waits on one that is
+// not running.  Returns num_tiles when every group is taken.
+__device__ __forceinline__ uint32_t take_grouped_tile(const mgpuMultiPassArgs& a) {
+    const uint32_t groups = a.num_xcds;
+    uint32_t g = blockIdx.x % groups;
+    for (uint32_t k = 0; k < groups; ++k) {
+        const uint32_t b = a.xcd_tile_begin[g], e = a.xcd_tile_begin[g + 1];
+        if (b < e) {
+            const uint32_t x = __hip_atomic_fetch_add(a.tile_counter + g, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            if (x < e - b) return b + x;
+        }
+        g = g + 1 == groups ? 0u : g + 1;
+    }
+    return a.num_tiles;
+}
+
 // Striped load of one array of a tile: slot j of lane l of wave w holds
 // element tile_base + w*64*IPT + j*64 + l.  Raw buffer loads with the array
 // length as the range: the hardware returns 0 for elements at or past n, so a
@@ -921,9 +945,14 @@ __device__ void rank_scatter_body(const Args& a) {
         // away), so a tile only ever waits on tiles that are already running.
         if (tid == 0) {
             uint32_t t = seg_tiles;  // a workgroup of an XCD without a segment takes none
-            if (seg_tiles > 0)
+            bool grouped = false;
+            if constexpr (MULTI && !XSEG) grouped = a.num_xcds > 1;
+            if (grouped) {
+                if constexpr (MULTI && !XSEG) t = take_grouped_tile(a);
+            } else if (seg_tiles > 0) {
                 t = __hip_atomic_fetch_add(a.tile_counter + seg, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
+            }
             s_tile[0] = t;
             s_tile[1] = MULTI && t < seg_tiles ? find_array<MULTI>(a, seg_first + t) : 0u;
         }
@@ -1287,7 +1316,9 @@ __device__ void packed_scatter_body(const mgpuMultiPassArgs& a) {
     while (true) {
         if (tid == 0) {
             uint32_t t = xtiles;
-            if (xtiles > 0)
+            if (!XSEG && a.num_xcds > 1)
+                t = take_grouped_tile(a);
+            else if (xtiles > 0)
                 t = __hip_atomic_fetch_add(a.tile_counter + xslot, 1u, __ATOMIC_RELAXED,
                                            __HIP_MEMORY_SCOPE_AGENT);
             s_tile = t;

@@ -26,6 +26,9 @@ namespace {
 // 0.764 ms; 4096 in [1, 2^16], 2.7 tiles each, 0.673 -> 0.729 ms).
 constexpr uint64_t kXcdMinTiles = 4;
 
+// Tile counters of a one-chain-per-segment launch (groups of segments).
+constexpr long kTileCounterGroups = 8;
+
 // Longest segment the device-wide count reads with one workgroup, writing
 // its digit starts directly (a 2^18-key segment: 1 MB, a few microseconds).
 constexpr uint32_t kDirectCountMax = 1u << 18;
@@ -356,6 +359,30 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         P.wideXcds = 1;
         P.xcdTileBegin.clear();
         std::vector<size_t> groupEnd(1, P.wide.size());  // end of each XCD's segments
+        // Segments into G groups, largest first to the group with the least
+        // work, laid out group by group (P.wide reordered, groupEnd and the
+        // groups' first tiles filled in).
+        auto group_by_load = [&](uint32_t G) {
+            std::vector<uint32_t> order(P.wide);
+            std::stable_sort(order.begin(), order.end(),
+                             [&](uint32_t a, uint32_t b) { return segs[a].n > segs[b].n; });
+            std::vector<std::vector<uint32_t>> group(G);
+            std::vector<uint64_t> load(G, 0);
+            for (uint32_t k : order) {
+                const uint32_t x =
+                    (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+                group[x].push_back(k);
+                load[x] += (segs[k].n + unit - 1) / unit;
+            }
+            P.wide.clear();
+            groupEnd.clear();
+            P.xcdTileBegin.assign(1, 0);
+            for (uint32_t x = 0; x < G; ++x) {
+                P.wide.insert(P.wide.end(), group[x].begin(), group[x].end());
+                groupEnd.push_back(P.wide.size());
+                P.xcdTileBegin.push_back(P.xcdTileBegin.back() + (uint32_t)load[x]);
+            }
+        };
         const uint32_t X = e->xcdSegs;
         const bool xcdKernels = P.wideVariant->multiX[MGPU_VALUES_NONE] != nullptr &&
                                 (packMode != 1 || P.wideVariant->packedX[MGPU_VALUES_NONE] != nullptr);
@@ -371,26 +398,23 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
             if (force && (force[0] == '0' || force[0] == '1') && force[1] == 0)
                 use = force[0] == '1' && most * 4 * X <= total;
             if (use) {
-                std::vector<uint32_t> order(P.wide);
-                std::stable_sort(order.begin(), order.end(),
-                                 [&](uint32_t a, uint32_t b) { return segs[a].n > segs[b].n; });
-                std::vector<std::vector<uint32_t>> group(X);
-                std::vector<uint64_t> load(X, 0);
-                for (uint32_t k : order) {
-                    const uint32_t x = (uint32_t)(std::min_element(load.begin(), load.end()) -
-                                                  load.begin());
-                    group[x].push_back(k);
-                    load[x] += (segs[k].n + unit - 1) / unit;
-                }
-                P.wide.clear();
-                groupEnd.clear();
-                P.xcdTileBegin.push_back(0);
-                for (uint32_t x = 0; x < X; ++x) {
-                    P.wide.insert(P.wide.end(), group[x].begin(), group[x].end());
-                    groupEnd.push_back(P.wide.size());
-                    P.xcdTileBegin.push_back(P.xcdTileBegin.back() + (uint32_t)load[x]);
-                }
+                group_by_load(X);
                 P.wideXcds = X;
+            }
+        }
+        // One chain per segment: the segments still go into groups (largest
+        // first, to the least loaded), each group's tiles handed out by a
+        // counter of its own (MGPUSORT_SEG_GROUPS, default 8; 1 = one
+        // counter for every tile).
+        P.wideGroups = 1;
+        if (P.wideXcds == 1 && P.wide.size() > 1) {
+            const char* v = std::getenv("MGPUSORT_SEG_GROUPS");
+            const long want = v ? std::strtol(v, nullptr, 10) : kTileCounterGroups;
+            const uint32_t G = (uint32_t)std::max<long>(
+                1, std::min<long>({want, (long)MGPU_MAX_SEGS, (long)P.wide.size()}));
+            if (G > 1) {
+                group_by_load(G);
+                P.wideGroups = G;
             }
         }
         P.widePacked = P.wideVariant->packed[MGPU_VALUES_NONE] != nullptr &&
@@ -459,7 +483,7 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
                 tileSeg[2 * t + 1] = startMask | (live << 16) | ((begin[j] < c0 ? 1u : 0u) << 24);
             }
             P.wideTiles = (uint32_t)tiles;
-            if (P.wideXcds > 1) P.xcdTileBegin = xb;
+            if (P.wideXcds > 1 || P.wideGroups > 1) P.xcdTileBegin = xb;
         } else {
             tileBegin.push_back(0);
             for (size_t k = 0; k < P.wide.size(); ++k) {
@@ -683,6 +707,10 @@ sortStatus_t mgpu_enqueue_wide(SegPlanData& P, SortLane& lane) {
                              P.widePacked ? P.wideVariant->packedX[mode] : P.wideVariant->multiX[mode],
                              g, P.wideVariant->blockThreads, &pa);
         } else {
+            if (P.wideGroups > 1) {
+                pa.xcd_tile_begin = P.devXcdTileBegin;
+                pa.num_xcds = P.wideGroups;
+            }
             st = mgpu_launch(e, lane.stream,
                              P.widePacked ? P.wideVariant->packed[mode] : P.wideVariant->multi[mode],
                              P.wideTiles, P.wideVariant->blockThreads, &pa);

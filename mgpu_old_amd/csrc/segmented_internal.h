@@ -107,6 +107,9 @@ struct SegPlanData {
     // grouped by XCD, XCD x holding global tiles [xcdTileBegin[x],
     // xcdTileBegin[x + 1]).
     uint32_t wideXcds = 1;
+    // One chain per segment (wideXcds == 1): groups of segments whose tiles
+    // have a counter each (wideGroups > 1), laid out the same way.
+    uint32_t wideGroups = 1;
     std::vector<uint32_t> xcdTileBegin;
     uint32_t* devXcdTileBegin = nullptr;
     void* devBlock = nullptr;  // owned allocation (null when an arena holds the arrays)

@@ -215,7 +215,7 @@ def test_index_mode_needs_both_value_sides(engine, eb):
 
 
 @pytest.mark.parametrize("layout", ["xcd", "one_chain", "small_pieces", "wave_count",
-                                    "dominant_segment", "unpacked", "packed_xcd"])
+                                    "dominant_segment", "unpacked", "packed_xcd", "one_counter"])
 def test_device_wide_layouts(oracle, layout):
     """The device-wide passes of a segmented sort in each layout: lookback
     chains per XCD with whole segments grouped by XCD (MGPUSORT_SEG_XCD=1),
@@ -227,8 +227,10 @@ def test_device_wide_layouts(oracle, layout):
     after the count), the per-wave count kernel instead of the lane-column
     one (MGPUSORT_SEG_COUNT=wave).  Packed tiles (segments back to back,
     MGPUSORT_SEG_PACK=1) with one chain per segment or chains per XCD, and
-    tiles of each segment's own (MGPUSORT_SEG_PACK=0).  Every segment
-    bit-exact against the oracle."""
+    tiles of each segment's own (MGPUSORT_SEG_PACK=0); with one chain per
+    segment the tiles come from a counter per group of segments, or from
+    one counter (MGPUSORT_SEG_GROUPS=1).  Every segment bit-exact against
+    the oracle."""
     import torch
     env = {"MGPUSORT_XCD": "0"} if layout == "one_chain" else {}
     eng = make_engine(env)
@@ -239,7 +241,8 @@ def test_device_wide_layouts(oracle, layout):
                               "MGPUSORT_SEG_PACK": "1"},
              "wave_count": {"MGPUSORT_SEG_COUNT": "wave"},
              "unpacked": {"MGPUSORT_SEG_PACK": "0", "MGPUSORT_SEG_XCD": "0"},
-             "packed_xcd": {"MGPUSORT_SEG_PACK": "1", "MGPUSORT_SEG_XCD": "1"}}.get(layout, {})
+             "packed_xcd": {"MGPUSORT_SEG_PACK": "1", "MGPUSORT_SEG_XCD": "1"},
+             "one_counter": {"MGPUSORT_SEG_GROUPS": "1", "MGPUSORT_SEG_XCD": "0"}}.get(layout, {})
     saved = {k: os.environ.get(k) for k in knobs}
     try:
         os.environ.update(knobs)
