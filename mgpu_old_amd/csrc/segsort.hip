@@ -387,7 +387,11 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
     for (uint32_t i = tid; i < (uint32_t)(W * SegShared<RB, IPT, BLOCK>::RADIX); i += BLOCK)
         (&sh.match[0][0])[i] = 0;
 
-    // Workgroup phase: block and global segments, largest first.
+    // Workgroup phase: block and global segments, largest first, from a
+    // shared cursor.  (Dealt round-robin like the small ones below, they
+    // measured slower where they are many: 20000 segments log-uniform in
+    // [2^6, 2^12] 0.44 -> 0.68 ms, since a workgroup that starts late still
+    // holds a full share; an atomic per segment of this size is cheap.)
     while (true) {
         if (tid == 0)
             sh.item = __hip_atomic_fetch_add(&a.counters[0], 1u, __ATOMIC_RELAXED,
