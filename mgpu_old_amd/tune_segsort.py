@@ -10,7 +10,8 @@ creation (kernelPath + "mgpusort_seg_tuning.txt"):
      segment takes them, the tile shape and whether tiles are packed
      (segments back to back) or one segment's own, timed with the bucket
      rows of step 1 in force (the count piece size only matters for segments
-     above 2^18 keys, which this batch does not have: it stays at 2^14).
+     above 2^18 keys, which this batch does not have: it stays at 2^16, the
+     better of 2^14 and 2^16 on the sort server's batches).
 
 It plays the role of the reference's offline pass-table generator
 (sort/src/tablegen/tablegen.cpp:9-64 over measured rates in throughputs.txt),
@@ -34,8 +35,12 @@ BUCKETS = [(0, 6), (6, 8), (8, 10), (10, 12), (12, 13)]
 VARIANTS = [f"r{rb}i{ipt}b{b}" for rb in range(4, 9) for ipt in (8, 16) for b in (128, 256, 512)
             if b >= (1 << rb)]
 WIDE_MIN = [1025, 2049, 4097]
-WIDE_CHUNK = [1 << 14]
-WIDE_TILE = ["small", "large"]
+WIDE_CHUNK = [1 << 16]
+# "auto" picks the small tile for this batch's segments (under four large
+# tiles on average) and the large one for batches of larger segments, which
+# a fixed "small" would slow down (the sort server's batches: 47 against 61
+# Gkeys/s); "large" is the alternative measured here.
+WIDE_TILE = ["auto", "large"]
 WIDE_PACK = ["packed", "unpacked"]
 
 
