@@ -151,6 +151,10 @@ struct sortEngine_d {
     hipModule_t segModule = nullptr;
     hipFunction_t segFn[kSegVariants] = {};
     int segOccupancy[kSegVariants] = {};
+    // The same variants without the routine for segments larger than a
+    // workgroup's tile (mgpu_segsort_*_ng): fewer registers, more waves.
+    hipFunction_t segFnTile[kSegVariants] = {};
+    int segOccupancyTile[kSegVariants] = {};
     std::vector<SegTuneEntry> segTable;
     SegWideTune segWide;
 

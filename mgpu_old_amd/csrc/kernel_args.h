@@ -179,9 +179,14 @@ struct mgpuSegArgs {
     uint32_t num_values;   // value arrays carried (index generation counts 1)
     uint32_t index_mode;   // generate source indices into vals[0]
     uint32_t result_side;  // 0 or 1
+    // Passes over bits [first_bit, first_bit + num_bits): num_passes digits,
+    // the narrower ones first, as sortPlanPasses splits the field (width
+    // num_bits / num_passes, one more for the last num_bits % num_passes).
+    // Derived in the kernel rather than read from per-pass arrays: indexing
+    // an argument array by the pass put the whole argument block in scratch.
     uint32_t num_passes;   // <= MGPU_SEG_MAX_PASSES
-    uint32_t shift[MGPU_SEG_MAX_PASSES];
-    uint32_t mask[MGPU_SEG_MAX_PASSES];
+    uint32_t first_bit;
+    uint32_t num_bits;
     // 1: rank by LDS atomic adds, which the engine checked are served in
     // ascending lane order; 0: by match masks.
     uint32_t rank1;

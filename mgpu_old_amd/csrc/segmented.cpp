@@ -112,18 +112,21 @@ sortStatus_t mgpu_seg_load(sortEngine_d* e) {
     for (int v = 0; v < kSegVariants; ++v) {
         const int rb = seg_variant_bits(v), ipt = seg_variant_ipt(v), block = seg_variant_block(v);
         if (!seg_variant_valid(rb, ipt, block)) continue;
-        char name[64];
-        std::snprintf(name, sizeof(name), "mgpu_segsort_r%d_i%d_b%d", rb, ipt, block);
-        hipFunction_t f = nullptr;
-        if (hipModuleGetFunction(&f, e->segModule, name) != hipSuccess)
-            return SORT_STATUS_KERNEL_ERROR;
-        e->kernelNames[f] = name;
-        int blocks = 0;
-        if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, 0) !=
-                hipSuccess || blocks < 1)
-            blocks = 1;
-        e->segFn[v] = f;
-        e->segOccupancy[v] = blocks;
+        for (int tileOnly = 0; tileOnly < 2; ++tileOnly) {
+            char name[64];
+            std::snprintf(name, sizeof(name), "mgpu_segsort_r%d_i%d_b%d%s", rb, ipt, block,
+                          tileOnly ? "_ng" : "");
+            hipFunction_t f = nullptr;
+            if (hipModuleGetFunction(&f, e->segModule, name) != hipSuccess)
+                return SORT_STATUS_KERNEL_ERROR;
+            e->kernelNames[f] = name;
+            int blocks = 0;
+            if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, f, block, 0) !=
+                    hipSuccess || blocks < 1)
+                blocks = 1;
+            (tileOnly ? e->segFnTile : e->segFn)[v] = f;
+            (tileOnly ? e->segOccupancyTile : e->segOccupancy)[v] = blocks;
+        }
     }
 
     // Tuning table: kernelPath + "mgpusort_seg_tuning.txt" when present
@@ -311,11 +314,16 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         L.numBig = (uint32_t)g.big.size();
         L.numSmall = (uint32_t)g.small.size();
         const int rb = seg_variant_bits(v);
-        L.fn = e->segFn[v];
+        // The kernel without the global-memory routine when every block-list
+        // segment fits a workgroup's tile (the list is largest first).
+        const uint32_t tileKeys = (uint32_t)(seg_variant_block(v) * seg_variant_ipt(v));
+        const bool tileOnly = g.big.empty() || segs[g.big.front()].n <= tileKeys;
+        L.fn = tileOnly ? e->segFnTile[v] : e->segFn[v];
         L.block = (uint32_t)seg_variant_block(v);
         const uint32_t wavesPerGroup = L.block / 64;
         const uint32_t work = L.numBig + (L.numSmall + wavesPerGroup - 1) / wavesPerGroup;
-        const uint32_t maxGrid = (uint32_t)(e->segOccupancy[v] * e->numCUs);
+        const uint32_t maxGrid =
+            (uint32_t)((tileOnly ? e->segOccupancyTile[v] : e->segOccupancy[v]) * e->numCUs);
         L.grid = std::max(1u, std::min(work, maxGrid));
         L.passes = mgpu_plan_passes(P.numBits, rb, L.widths);
         P.launches.push_back(L);
@@ -755,13 +763,9 @@ sortStatus_t mgpu_seg_enqueue(SegPlanData& P, SortLane& lane) {
         a.index_mode = P.valueCount == -1;
         a.result_side = (uint32_t)P.resultSide;
         a.num_passes = (uint32_t)L.passes;
+        a.first_bit = (uint32_t)P.firstBit;
+        a.num_bits = (uint32_t)P.numBits;
         a.rank1 = e->rankMethod == 1 ? 1u : 0u;
-        int bit = P.firstBit;
-        for (int p = 0; p < L.passes; ++p) {
-            a.shift[p] = (uint32_t)bit;
-            a.mask[p] = (1u << L.widths[p]) - 1u;
-            bit += L.widths[p];
-        }
         sortStatus_t st = mgpu_launch(e, ls, L.fn, L.grid, L.block, &a);
         if (st) return st;
     }

@@ -64,6 +64,15 @@ __device__ __forceinline__ uint32_t seg_rank(uint32_t d, uint32_t lane, uint32_t
     return __hip_atomic_fetch_add(&count[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Bit field of pass p: shift and mask (the split of mgpuSegArgs).
+__device__ __forceinline__ void pass_field(const mgpuSegArgs& a, uint32_t p, uint32_t& shift,
+                                           uint32_t& mask) {
+    const uint32_t np = a.num_passes, base = a.num_bits / np, extra = a.num_bits % np;
+    const uint32_t wide_from = np - extra;  // passes from here take base + 1 bits
+    shift = a.first_bit + p * base + (p > wide_from ? p - wide_from : 0u);
+    mask = (1u << (base + (p >= wide_from ? 1u : 0u))) - 1u;
+}
+
 // Shared memory of one workgroup of BLOCK threads (W waves).
 template <int RB, int IPT, int BLOCK>
 struct SegShared {
@@ -135,7 +144,8 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
         key[j] = idx[j] < n ? ki[idx[j]] : 0u;
     }
     for (uint32_t p = 0; p < a.num_passes; ++p) {
-        const uint32_t shift = a.shift[p], mask = a.mask[p];
+        uint32_t shift, mask;
+        pass_field(a, p, shift, mask);
 #pragma unroll
         for (int i = 0; i < DPL; ++i)
             if (lane * DPL + i < (uint32_t)RADIX) count[lane * DPL + i] = 0;
@@ -203,7 +213,8 @@ __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared
         key[j] = idx[j] < n ? ki[idx[j]] : 0u;
     }
     for (uint32_t p = 0; p < a.num_passes; ++p) {
-        const uint32_t shift = a.shift[p], mask = a.mask[p];
+        uint32_t shift, mask;
+        pass_field(a, p, shift, mask);
         for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&sh.count[0][0])[i] = 0;
         __syncthreads();
         uint32_t rank[IPT];
@@ -266,9 +277,12 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
     __syncthreads();
     for (uint32_t i = tid; i < n; i += BLOCK) {
         const uint32_t k = s.keys[0][i];
-        for (uint32_t p = 0; p < np; ++p)
-            __hip_atomic_fetch_add(&sh.base[p][(k >> a.shift[p]) & a.mask[p]], 1u,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        for (uint32_t p = 0; p < np; ++p) {
+            uint32_t shift, mask;
+            pass_field(a, p, shift, mask);
+            __hip_atomic_fetch_add(&sh.base[p][(k >> shift) & mask], 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
     __syncthreads();
     for (uint32_t p = 0; p < np; ++p) {
@@ -280,7 +294,8 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
     const uint32_t nv = a.num_values;
     for (uint32_t p = 0; p < np; ++p) {
         const uint32_t in = p & 1, out = in ^ 1;
-        const uint32_t shift = a.shift[p], mask = a.mask[p];
+        uint32_t shift, mask;
+        pass_field(a, p, shift, mask);
         const uint32_t* ki = s.keys[in];
         uint32_t* ko = s.keys[out];
         if (tid < (uint32_t)RADIX) sh.run[tid] = sh.base[p][tid];
@@ -378,7 +393,13 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
     }
 }
 
-template <int RB, int IPT, int BLOCK>
+// GLOBAL: the kernel carries the one-workgroup routine for segments larger
+// than a workgroup's LDS tile.  That routine sets the kernel's register
+// count (r8, 16 keys per thread, 512 threads: 221 VGPRs and 96 bytes of
+// scratch with it, 141 and none without; 8 keys per thread: 132 -> 89), so
+// launches whose block-list segments all fit the tile take the kernel
+// without it, which fits more waves per SIMD.
+template <int RB, int IPT, int BLOCK, bool GLOBAL>
 __device__ void segsort_body(const mgpuSegArgs& a) {
     static_assert(BLOCK >= (1 << RB), "a thread per digit in the block scans");
     constexpr int W = BLOCK / 64;
@@ -401,8 +422,13 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
         __syncthreads();
         if (item >= a.num_big) break;
         const mgpuSegDesc& s = a.segs[a.big_list[item]];
-        if (s.n <= (uint32_t)SegShared<RB, IPT, BLOCK>::TILE) block_sort<RB, IPT, BLOCK>(a, s, sh, tid, lane, wave);
-        else global_sort<RB, IPT, BLOCK>(a, s, sh, tid, lane, wave);
+        if (s.n <= (uint32_t)SegShared<RB, IPT, BLOCK>::TILE) {
+            block_sort<RB, IPT, BLOCK>(a, s, sh, tid, lane, wave);
+        } else if constexpr (GLOBAL) {
+            global_sort<RB, IPT, BLOCK>(a, s, sh, tid, lane, wave);
+        } else if (tid == 0) {
+            mgpu::set_error(a.error_flag, 7u);  // the host never sends one here
+        }
         __syncthreads();
     }
 
@@ -428,11 +454,16 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
 }  // namespace
 
 // Kernel variants: digit width RB (4..8 bits) x keys per thread IPT (8, 16)
-// x workgroup size BLOCK (128 for digits up to 7 bits, 256, 512).
+// x workgroup size BLOCK (128 for digits up to 7 bits, 256, 512), each with
+// the one-workgroup global-memory routine and without it (_ng).
 #define MGPU_SEGSORT(RB, IPT, B)                                                                \
     extern "C" __global__ void __launch_bounds__(B)                                            \
         mgpu_segsort_r##RB##_i##IPT##_b##B(mgpuSegArgs a) {                                    \
-        segsort_body<RB, IPT, B>(a);                                                           \
+        segsort_body<RB, IPT, B, true>(a);                                                     \
+    }                                                                                          \
+    extern "C" __global__ void __launch_bounds__(B)                                            \
+        mgpu_segsort_r##RB##_i##IPT##_b##B##_ng(mgpuSegArgs a) {                               \
+        segsort_body<RB, IPT, B, false>(a);                                                    \
     }
 #define MGPU_SEGSORT_BLOCKS(RB, IPT)                                                           \
     MGPU_SEGSORT(RB, IPT, 256)                                                                 \

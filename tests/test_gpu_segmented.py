@@ -58,7 +58,8 @@ def test_size_classes_per_variant(engine, oracle, variant, wide_min):
     """Every size class with every kernel variant.  By default segments above
     4096 keys take the device-wide passes; MGPUSORT_SEG_WIDE_MIN=65537 keeps
     segments up to 2^16 keys in the persistent kernel, whose one-workgroup
-    global routine then sorts those above its LDS capacity."""
+    global routine then sorts those above its LDS capacity (the kernel built
+    without that routine, _ng, takes launches that need none)."""
     old = os.environ.get("MGPUSORT_SEG_VARIANT")
     old_wide = os.environ.get("MGPUSORT_SEG_WIDE_MIN")
     if variant:
@@ -71,10 +72,22 @@ def test_size_classes_per_variant(engine, oracle, variant, wide_min):
         keys = workloads.uniform_keys(n, seed=41)
         keys[: n // 3] &= np.uint32(0x0F0F)  # ties
         vals = [np.arange(n, dtype=np.uint32)]
+        engine.enable_timing(True)
+        engine.read_timing()
         g = _run_segmented(engine, keys, vals, 1, starts)
+        names = {k for k in engine.read_timing() if k.startswith("mgpu_segsort")}
+        engine.enable_timing(False)
         _check_segments(oracle, keys, vals, 1, starts, g.keys(), g.values(), 0, 32)
         assert g.data.parity == 0
+        # Segments above a workgroup's tile need the kernel with the global
+        # routine; without any (the shipped table's variants: 512 threads,
+        # every tile at least 4096 keys), every launch takes the one without.
+        if wide_min:
+            assert any(not k.endswith("_ng") for k in names), names
+        elif variant is None:
+            assert names and all(k.endswith("_ng") for k in names), names
     finally:
+        engine.enable_timing(False)
         if old is None:
             os.environ.pop("MGPUSORT_SEG_VARIANT", None)
         else:
@@ -317,7 +330,8 @@ def test_seg_table_dimensions_honoured(tmp_path, oracle):
         names = set(eng.read_timing())
         eng.enable_timing(False)
         _check_segments(oracle, keys, vals, 1, starts, g.keys(), g.values(), 0, 32)
-        seg_kernels = {k for k in names if k.startswith("mgpu_segsort")}
+        # Kernels with or without the global-memory routine (_ng) count alike.
+        seg_kernels = {k.removesuffix("_ng") for k in names if k.startswith("mgpu_segsort")}
         assert seg_kernels == {"mgpu_segsort_r5_i8_b128", "mgpu_segsort_r8_i16_b512",
                                "mgpu_segsort_r7_i16_b256"}, names
         scatter = {k for k in names if k.startswith("mgpu_multi_packed")}
