@@ -223,9 +223,10 @@ int SORTAPI sortPlanPasses(int numBits, int* widths);
  * up exactly as sortArray would leave it if called on that segment alone with
  * the same bit range and value count: stable by the bit field, values
  * following, the result on side (sortPlanPasses(endBit - firstBit) & 1) of its
- * buffers.  Segments of up to 4096 elements are sorted by one wave or one
- * workgroup each (digit width and keys per thread per size bucket from the
- * tuning table, kernelPath + "mgpusort_seg_tuning.txt"); larger ones run the
+ * buffers.  Segments below the tuning table's device-wide threshold (4097
+ * elements in the shipped kernelPath + "mgpusort_seg_tuning.txt") are sorted
+ * by one wave or one workgroup each (digit width, keys per thread and
+ * workgroup size per size bucket from the same table); larger ones run the
  * device-wide passes, all of them in one launch per pass.  With
  * SORT_VALUE_INDEX both sides of values[0] must be given. */
 
@@ -324,8 +325,11 @@ sortStatus_t SORTAPI sortCheckDevice(sortEngine_t engine);
 
 /* What the engine chose at creation, and the last device inconsistency code
  * (0 none; 1 a lookback wait gave up; 2 a tile counter left dirty; 4 a
- * scatter offset out of range; 5 a segment with no workgroup on its XCD)
- * without clearing it.  No counterpart in the reference. */
+ * scatter offset out of range; 5 a segment with no workgroup on its XCD;
+ * 6 a packed tile with more segments than it can hold; 7 a segment larger
+ * than the tile of a kernel built without the global-memory routine: both
+ * only from a broken plan) without clearing it.  No counterpart in the
+ * reference. */
 typedef struct {
     int device;
     int numCUs;
