@@ -22,6 +22,18 @@ __device__ __forceinline__ void st_agent(uint32_t* p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A pointer read from memory (an array's address in a segment descriptor) is
+// a generic pointer to the compiler, which accesses it with flat
+// instructions.  A flat access counts on the LDS counter as well as the
+// memory one, so every later wait for an LDS result also waits for it (a
+// load's data, a store's write), and a wait for its data also drains the LDS
+// atomics in flight.  to_global gives the same address as a global-memory
+// pointer, accessed with global instructions.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* to_global(T* p) {
+    return (__attribute__((address_space(1))) T*)p;
+}
+
 // Records a failed self-check in the launch's error word, which lives in
 // host memory mapped into the device (system scope: the host reads it).
 __device__ __forceinline__ void set_error(uint32_t* flag, uint32_t code) {

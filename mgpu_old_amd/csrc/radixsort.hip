@@ -30,6 +30,7 @@
 #include <stdint.h>
 
 #include "kernel_args.h"
+#include "radix_device.h"
 
 namespace {
 
@@ -439,6 +440,9 @@ __device__ void digit_scan_body(const mgpuScanArgs& a) {
 // ---------------------------------------------------------------------------
 // Device-wide passes over several arrays: digit_count per chunk of an array
 // (one chunk per workgroup), digit_scan per array.
+// Keys are read through to_global pointers (a descriptor's pointer would be
+// read with flat loads).
+
 template <int BLOCK>
 __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
     constexpr int W = BLOCK / 64;
@@ -457,7 +461,8 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
     __syncthreads();
     const uint32_t* ch = a.chunks + 4 * blockIdx.x;
     const uint32_t sg = ch[0], begin = ch[1], end = ch[2];
-    const uint32_t* keys = a.segs[sg].keys[0];
+    const uint32_t* kp = a.segs[sg].keys[0];
+    const auto keys = mgpu::to_global(kp);
     const uint32_t np = a.num_passes;
     uint32_t (*h)[RADIX] = s_hist[wave];
     auto count_key = [&](uint32_t k) {
@@ -468,12 +473,12 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
                                        __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     // Scalar head up to 16-byte alignment, vector body, scalar tail.
-    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(keys + begin) >> 2) & 3);
+    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(kp + begin) >> 2) & 3);
     const uint32_t vbeg = min(end, begin + ((4 - mis) & 3));
     const uint32_t nvec = (end - vbeg) >> 2;
     for (uint32_t i = begin + tid; i < vbeg; i += BLOCK) count_key(keys[i]);
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4* v = reinterpret_cast<const u32x4*>(keys + vbeg);
+    const auto v = mgpu::to_global(reinterpret_cast<const u32x4*>(kp + vbeg));
     // A 16K-key chunk is 16 vectors per thread: all of them in flight at once
     // (4096 segments in [1, 2^16]: 78 -> 73 us per launch).
     constexpr int UNROLL = 16;
@@ -528,7 +533,8 @@ __device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
     __syncthreads();
     const uint32_t* ch = a.chunks + 4 * blockIdx.x;
     const uint32_t sg = ch[0], begin = ch[1], end = ch[2];
-    const uint32_t* keys = a.segs[sg].keys[0];
+    const uint32_t* kp = a.segs[sg].keys[0];
+    const auto keys = mgpu::to_global(kp);
     const uint32_t np = a.num_passes;
     auto add = [&](uint32_t p, uint32_t k) {
         const uint32_t d = (k >> a.shift[p]) & a.mask[p];
@@ -541,12 +547,12 @@ __device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
             if ((uint32_t)p < np) add(p, k);
     };
     // Scalar head up to 16-byte alignment, vector body, scalar tail.
-    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(keys + begin) >> 2) & 3);
+    const uint32_t mis = (uint32_t)((reinterpret_cast<uintptr_t>(kp + begin) >> 2) & 3);
     const uint32_t vbeg = min(end, begin + ((4 - mis) & 3));
     const uint32_t nvec = (end - vbeg) >> 2;
     for (uint32_t i = begin + tid; i < vbeg; i += BLOCK) count_key(keys[i]);
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    const u32x4* v = reinterpret_cast<const u32x4*>(keys + vbeg);
+    const auto v = mgpu::to_global(reinterpret_cast<const u32x4*>(kp + vbeg));
     constexpr int UNROLL = 16;  // a 64K-key piece: every vector of a thread in flight
     for (uint32_t i0 = tid; i0 < nvec; i0 += UNROLL * BLOCK) {
         u32x4 q[UNROLL];

@@ -88,6 +88,21 @@ bool count_cols() {
     return !(v && std::strcmp(v, "wave") == 0);
 }
 
+// Workgroups per CU of a persistent kernel that runs beside device-wide
+// passes: one (at most its occupancy), so that every CU keeps room for the
+// count and pass workgroups, which are on the critical path, instead of the
+// persistent grid filling the device first.  Measured by plan replays, key +
+// value, occupancy -> one per CU: 4096 segments log-uniform in [1, 2^16]
+// 0.585 -> 0.577 ms; 2000 in [2^10, 2^14] 0.284 -> 0.266 ms; 8000 in [2^8,
+// 2^15] 1.21 -> 1.17 ms; 16000 in [1, 2^13] 0.415 -> 0.346 ms; 500 in [2^4,
+// 2^17] 0.255 ms either way (two per CU: as occupancy).
+// MGPUSORT_SEG_MIX_WGS=<n> overrides it (measurements).
+int mixed_workgroups_per_cu() {
+    const char* v = std::getenv("MGPUSORT_SEG_MIX_WGS");
+    const long x = v ? std::strtol(v, nullptr, 10) : 0;
+    return x > 0 ? (int)std::min<long>(x, 64) : 1;
+}
+
 const SegTuneEntry kDefaultTable[] = {
     // max log2(n), radix bits, items per thread, workgroup size
     {6, 8, 8, 256}, {10, 8, 16, 256}, {12, 8, 16, 256}, {31, 8, 16, 256},
@@ -297,6 +312,11 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         else g.big.push_back((uint32_t)i);
     }
 
+    // Persistent kernels forked beside device-wide passes take at most
+    // mixed_workgroups_per_cu() workgroups per CU.
+    const bool mixed = !P.wide.empty() && !lane.noSide;
+    const int mixedWgs = mixed_workgroups_per_cu();
+
     // Device arrays: descriptors, then per group [big list | small list],
     // then 4 counters per group.
     std::vector<uint32_t> lists;
@@ -322,8 +342,9 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         L.block = (uint32_t)seg_variant_block(v);
         const uint32_t wavesPerGroup = L.block / 64;
         const uint32_t work = L.numBig + (L.numSmall + wavesPerGroup - 1) / wavesPerGroup;
-        const uint32_t maxGrid =
-            (uint32_t)((tileOnly ? e->segOccupancyTile[v] : e->segOccupancy[v]) * e->numCUs);
+        int occ = tileOnly ? e->segOccupancyTile[v] : e->segOccupancy[v];
+        if (mixed) occ = std::min(occ, mixedWgs);
+        const uint32_t maxGrid = (uint32_t)(occ * e->numCUs);
         L.grid = std::max(1u, std::min(work, maxGrid));
         L.passes = mgpu_plan_passes(P.numBits, rb, L.widths);
         P.launches.push_back(L);
@@ -437,15 +458,27 @@ sortStatus_t mgpu_seg_build(sortEngine_d* e, SortLane& lane, const mgpuSegDesc* 
         P.wideDirectCount = count_cols() && longest <= kDirectCountMax &&
                             std::getenv("MGPUSORT_SEG_WIDE_CHUNK") == nullptr;
         const uint32_t pieceKeys = P.wideDirectCount ? longest : chunkKeys;
+        // Pieces longest first: workgroups start in about launch order, so
+        // the longest counts start first and the short ones fill in behind
+        // them (greedy longest-processing-time order).  4096 segments
+        // log-uniform in [1, 2^16]: count of the 1,007 device-wide ones
+        // 61 -> 51 us alone, replay 0.598 -> 0.586 ms.
+        struct Piece { uint32_t seg, begin, end; };
+        std::vector<Piece> pieces;
         for (size_t k = 0; k < P.wide.size(); ++k) {
             const mgpuSegDesc& d = segs[P.wide[k]];
             wideDescs.push_back(d);
-            for (uint32_t b = 0; b < d.n; b += pieceKeys) {
-                chunks.push_back((uint32_t)k);
-                chunks.push_back(b);
-                chunks.push_back(std::min(d.n, b + pieceKeys));
-                chunks.push_back(0);
-            }
+            for (uint32_t b = 0; b < d.n; b += pieceKeys)
+                pieces.push_back({(uint32_t)k, b, std::min(d.n, b + pieceKeys)});
+        }
+        std::stable_sort(pieces.begin(), pieces.end(), [](const Piece& x, const Piece& y) {
+            return x.end - x.begin > y.end - y.begin;
+        });
+        for (const Piece& c : pieces) {
+            chunks.push_back(c.seg);
+            chunks.push_back(c.begin);
+            chunks.push_back(c.end);
+            chunks.push_back(0);
         }
         if (P.widePacked) {
             // Wave chunks back to back, `waves` to a tile.  A segment that

@@ -98,20 +98,20 @@ __device__ __forceinline__ void emit(const mgpuSegArgs& a, const mgpuSegDesc& s,
                                      const uint32_t* key, const uint32_t* src, const uint32_t* idx,
                                      Sync sync) {
     const uint32_t side = a.result_side;
-    uint32_t* ko = s.keys[side];
+    const auto ko = mgpu::to_global(s.keys[side]);
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
         if (idx[j] < n) ko[idx[j]] = key[j];
     if (a.index_mode) {
-        uint32_t* vo = s.vals[0][side];
+        const auto vo = mgpu::to_global(s.vals[0][side]);
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
             if (idx[j] < n) vo[idx[j]] = src[j];
         return;
     }
     for (uint32_t v = 0; v < a.num_values; ++v) {
-        const uint32_t* vi = s.vals[v][0];
-        uint32_t* vo = s.vals[v][side];
+        const auto vi = mgpu::to_global(s.vals[v][0]);
+        const auto vo = mgpu::to_global(s.vals[v][side]);
         uint32_t val[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
@@ -136,7 +136,7 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
     uint64_t* match = sh.match[wave];
 
     uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
-    const uint32_t* ki = s.keys[0];
+    const auto ki = mgpu::to_global(s.keys[0]);
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         idx[j] = j * 64 + lane;
@@ -205,7 +205,7 @@ __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared
     constexpr int W = BLOCK / 64;
     const uint32_t n = s.n;
     uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
-    const uint32_t* ki = s.keys[0];
+    const auto ki = mgpu::to_global(s.keys[0]);
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         idx[j] = wave * 64 * IPT + j * 64 + lane;
@@ -276,7 +276,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
     for (uint32_t i = tid; i < np * RADIX; i += BLOCK) (&sh.base[0][0])[i] = 0;
     __syncthreads();
     for (uint32_t i = tid; i < n; i += BLOCK) {
-        const uint32_t k = s.keys[0][i];
+        const uint32_t k = mgpu::to_global(s.keys[0])[i];
         for (uint32_t p = 0; p < np; ++p) {
             uint32_t shift, mask;
             pass_field(a, p, shift, mask);
@@ -296,8 +296,8 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
         const uint32_t in = p & 1, out = in ^ 1;
         uint32_t shift, mask;
         pass_field(a, p, shift, mask);
-        const uint32_t* ki = s.keys[in];
-        uint32_t* ko = s.keys[out];
+        const auto ki = mgpu::to_global(s.keys[in]);
+        const auto ko = mgpu::to_global(s.keys[out]);
         if (tid < (uint32_t)RADIX) sh.run[tid] = sh.base[p][tid];
         for (uint32_t tb = 0; tb < n; tb += TILE) {
             for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&sh.count[0][0])[i] = 0;
@@ -354,7 +354,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
             }
             for (uint32_t v = 0; v < nv; ++v) {
                 const bool gen = a.index_mode && p == 0;
-                const uint32_t* vi = s.vals[v][in];
+                const auto vi = mgpu::to_global(s.vals[v][in]);
                 uint32_t val[IPT];
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
@@ -368,7 +368,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
                     if (i < n) sh.stage[pos[j]] = val[j];
                 }
                 __syncthreads();
-                uint32_t* vo = s.vals[v][out];
+                const auto vo = mgpu::to_global(s.vals[v][out]);
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t q = j * BLOCK + tid;
@@ -387,9 +387,12 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
     // The result is on side (np & 1); move it when the launch wants the other.
     if ((np & 1) != a.result_side) {
         const uint32_t from = np & 1, to = a.result_side;
-        for (uint32_t i = tid; i < n; i += BLOCK) s.keys[to][i] = s.keys[from][i];
+        const auto kt = mgpu::to_global(s.keys[to]);
+        const auto kf = mgpu::to_global(s.keys[from]);
+        for (uint32_t i = tid; i < n; i += BLOCK) kt[i] = kf[i];
         for (uint32_t v = 0; v < nv; ++v)
-            for (uint32_t i = tid; i < n; i += BLOCK) s.vals[v][to][i] = s.vals[v][from][i];
+            for (uint32_t i = tid; i < n; i += BLOCK)
+                mgpu::to_global(s.vals[v][to])[i] = mgpu::to_global(s.vals[v][from])[i];
     }
 }
 
