@@ -300,10 +300,13 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                                 count4(mine + p * PSTRIDE, q[u].x, q[u].y, q[u].z, q[u].w,
                                        a.shift[p], a.mask[p]);
                             } else {
-                                add(p, (q[u].x >> a.shift[p]) & a.mask[p]);
-                                add(p, (q[u].y >> a.shift[p]) & a.mask[p]);
-                                add(p, (q[u].z >> a.shift[p]) & a.mask[p]);
-                                add(p, (q[u].w >> a.shift[p]) & a.mask[p]);
+                                // The digit by one bit-field extract (mask =
+                                // 2^width - 1).
+                                const uint32_t w = (uint32_t)__builtin_popcount(a.mask[p]);
+                                add(p, __builtin_amdgcn_ubfe(q[u].x, a.shift[p], w));
+                                add(p, __builtin_amdgcn_ubfe(q[u].y, a.shift[p], w));
+                                add(p, __builtin_amdgcn_ubfe(q[u].z, a.shift[p], w));
+                                add(p, __builtin_amdgcn_ubfe(q[u].w, a.shift[p], w));
                             }
                         }
                     }
@@ -519,9 +522,10 @@ __device__ void multi_count_body(const mgpuMultiCountArgs& a) {
 __device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
     constexpr int BLOCK = 1024;
     constexpr uint32_t COLS = 32, ROWS = MGPU_MAX_PASSES * RADIX / 2;
-    __shared__ uint32_t s_hist[ROWS * COLS];  // 64 KB
+    __shared__ __attribute__((aligned(16))) uint32_t s_hist[ROWS * COLS];  // 64 KB
     const uint32_t tid = threadIdx.x, col = tid & (COLS - 1);
-    for (uint32_t i = tid; i < ROWS * COLS; i += BLOCK) s_hist[i] = 0;
+    for (uint32_t i = tid; i < ROWS * COLS / 4; i += BLOCK)
+        reinterpret_cast<uint4*>(s_hist)[i] = make_uint4(0u, 0u, 0u, 0u);
     {
         uint4* c = reinterpret_cast<uint4*>(a.clear);
         const uint64_t nvec = a.clear_words >> 2;
@@ -536,9 +540,21 @@ __device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
     const uint32_t* kp = a.segs[sg].keys[0];
     const auto keys = mgpu::to_global(kp);
     const uint32_t np = a.num_passes;
-    auto add = [&](uint32_t p, uint32_t k) {
-        const uint32_t d = (k >> a.shift[p]) & a.mask[p];
-        __hip_atomic_fetch_add(&s_hist[((p * RADIX + d) >> 1) * COLS + col], 1u << ((d & 1u) << 4),
+    // Pass p's word for key k: row d / 2 of the pass's RADIX / 2 (digit d =
+    // the field of `width` bits at `shift`), the parity of d picking the
+    // 16-bit half.  Both come from bit-field extracts: the row is the field
+    // without its low bit, the increment 1 or 1 << 16 by that bit.
+    uint32_t row_at[MGPU_MAX_PASSES], row_bits[MGPU_MAX_PASSES];
+#pragma unroll
+    for (int p = 0; p < MGPU_MAX_PASSES; ++p) {
+        const uint32_t width = (uint32_t)__builtin_popcount(a.mask[p]);
+        row_at[p] = a.shift[p] + 1;
+        row_bits[p] = width > 0 ? width - 1 : 0;
+    }
+    auto add = [&](int p, uint32_t k) {
+        const uint32_t row = __builtin_amdgcn_ubfe(k, row_at[p], row_bits[p]);
+        const uint32_t inc = __builtin_amdgcn_ubfe(k, a.shift[p], 1) * 0xFFFFu + 1u;
+        __hip_atomic_fetch_add(&s_hist[((uint32_t)p * (RADIX / 2) + row) * COLS + col], inc,
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     };
     auto count_key = [&](uint32_t k) {
@@ -562,24 +578,36 @@ __device__ void multi_count_cols_body(const mgpuMultiCountArgs& a) {
 #pragma unroll
         for (int u = 0; u < UNROLL; ++u) {
             if (i0 + u * BLOCK >= nvec) break;
-            count_key(q[u].x);
-            count_key(q[u].y);
-            count_key(q[u].z);
-            count_key(q[u].w);
+            // The pass test once per pass and vector, not per key.
+#pragma unroll
+            for (int p = 0; p < MGPU_MAX_PASSES; ++p) {
+                if ((uint32_t)p >= np) break;
+                add(p, q[u].x);
+                add(p, q[u].y);
+                add(p, q[u].z);
+                add(p, q[u].w);
+            }
         }
     }
     for (uint32_t i = vbeg + 4 * nvec + tid; i < end; i += BLOCK) count_key(keys[i]);
     __syncthreads();
-    // Thread (p, d) = (tid / 256, tid % 256) sums its digit's 32 columns,
-    // starting at column tid so that a half-wave reads 32 distinct banks.
+    // Thread (p, d) = (tid / 256, tid % 256) and its partner tid ^ 1 hold the
+    // two digits of row tid / 2: each sums half the row's columns, both
+    // halves of every word, and hands the partner the sum of the partner's
+    // half.  Reads start at column tid / 2 of the thread's 16 so that a
+    // half-wave reads 32 distinct banks.
     const uint32_t p = tid / RADIX, d = tid % RADIX;
-    uint32_t sum = 0;
+    const uint32_t h = tid & 1u, row = tid >> 1;
+    uint32_t lo = 0, hi = 0;
     if (p < np) {
-        const uint32_t row = (p * RADIX + d) >> 1, sh = (d & 1u) << 4;
 #pragma unroll
-        for (uint32_t c = 0; c < COLS; ++c)
-            sum += (s_hist[row * COLS + ((c + tid) & (COLS - 1))] >> sh) & 0xFFFFu;
+        for (uint32_t j = 0; j < COLS / 2; ++j) {
+            const uint32_t w = s_hist[row * COLS + h * (COLS / 2) + ((j + row) & (COLS / 2 - 1))];
+            lo += w & 0xFFFFu;
+            hi += w >> 16;
+        }
     }
+    const uint32_t sum = (h ? hi : lo) + (uint32_t)__shfl_xor((int)(h ? lo : hi), 1, 64);
     if (a.direct) {
         // The piece is the whole array: its digit starts, pass by pass (a
         // scan per group of 256 threads), with no global histogram and no
