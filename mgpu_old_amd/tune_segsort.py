@@ -11,7 +11,14 @@ creation (kernelPath + "mgpusort_seg_tuning.txt"):
      (segments back to back) or one segment's own, timed with the bucket
      rows of step 1 in force (the count piece size only matters for segments
      above 2^18 keys, which this batch does not have: it stays at 2^16, the
-     better of 2^14 and 2^16 on the sort server's batches).
+     better of 2^14 and 2^16 on the sort server's batches);
+  3. on the same batch, for each threshold of step 2, one variant for every
+     segment below it in place of the bucket rows of step 1, when faster: each bucket row
+     is a persistent launch of its own, and in a batch those launches run one
+     after another (each grid fills the device), so one launch whose tile
+     holds the largest such segment can beat rows that win bucket by bucket
+     (measured: the 3,089 segments of up to 4096 keys of that batch 0.116 ms
+     as three launches, 0.070 ms as one of r8i8b512).
 
 It plays the role of the reference's offline pass-table generator
 (sort/src/tablegen/tablegen.cpp:9-64 over measured rates in throughputs.txt),
@@ -162,6 +169,31 @@ def main():
                               flush=True)
         best_wide = min(wide, key=wide.get)
         report["wide"] = {"best": best_wide, "keys": n, "ms": {k: t * 1e3 for k, t in wide.items()}}
+
+        # 3. One launch for every size below the device-wide threshold, for
+        #    each threshold (with its best device-wide line of step 2).
+        single = {("buckets", best_wide): wide[best_wide]}
+        for m in WIDE_MIN:
+            line_m = min((line for line in wide if int(line.split()[1]) == m), key=wide.get)
+            top = max(0, (m - 2).bit_length())  # ceil(log2(m - 1))
+            above = "".join(f"{hi} {rb} {ipt} {b}\n" for hi, rb, ipt, b in rows if hi > top)
+            for v in VARIANTS:
+                rb, rest = v[1:].split("i")
+                ipt, block = (int(x) for x in rest.split("b"))
+                if ipt * block < (1 << top):
+                    continue  # the largest such segment would not fit the tile
+                text = f"{top} {rb} {ipt} {block}\n" + above
+                eng = engine_on(torch, tmp, text + line_m + "\n")
+                single[(text, line_m)] = time_plan(torch, eng, keys, vals, starts, args.reps)
+                eng.close()
+                print(f"[tune] 4096 segments ({n} keys) {line_m}, one launch of {v} up to 2^{top}: "
+                      f"{single[(text, line_m)] * 1e3:.3f} ms", flush=True)
+        best_rows, best_wide = min(single, key=single.get)
+        if best_rows != "buckets":
+            bucket_text = best_rows
+            rows = [tuple(int(x) for x in line.split()) for line in best_rows.splitlines()]
+        report["single"] = {"best": [best_rows, best_wide],
+                            "ms": {f"{k[0]}|{k[1]}": t * 1e3 for k, t in single.items()}}
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
