@@ -73,16 +73,21 @@ __device__ __forceinline__ void pass_field(const mgpuSegArgs& a, uint32_t p, uin
     mask = (1u << (base + (p >= wide_from ? 1u : 0u))) - 1u;
 }
 
-// Shared memory of one workgroup of BLOCK threads (W waves).
+// Shared memory of one workgroup of BLOCK threads (W waves).  The in-LDS
+// routines stage a slot's key and payload together, one 64-bit word; the
+// global routine uses the stage's first TILE 32-bit words.  Wave w's slice
+// (64 * IPT words from w * 64 * IPT) also holds, in its first RADIX words,
+// the wave's match-mask table of the match-mask ranking (match_table), which
+// the staging overwrites: that ranking clears it first (clear_match).
 template <int RB, int IPT, int BLOCK>
 struct SegShared {
     static constexpr int W = BLOCK / 64;
     static constexpr int RADIX = 1 << RB;
     static constexpr int TILE = BLOCK * IPT;
     static constexpr int MAXP = (32 + RB - 1) / RB;
-    uint32_t stage[TILE];
+    static_assert(64 * IPT >= RADIX, "a wave's stage slice holds its match-mask table");
+    uint64_t stage[TILE];
     uint32_t count[W][RADIX];
-    uint64_t match[W][RADIX];
     uint32_t base[MAXP][RADIX];  // global routine: per-pass digit starts
     uint32_t run[RADIX];         // global routine: running digit offsets
     uint32_t dst[RADIX];         // global routine: output base per digit for a tile
@@ -90,23 +95,48 @@ struct SegShared {
     uint32_t item;
 };
 
-// Writes the sorted keys and gathers every value array through the source
-// indices: out[pos] = in[src[pos]].  `sync` orders the gathers of the whole
-// group before any store (the result side may be the input side).
+template <int RB, int IPT, int BLOCK>
+__device__ __forceinline__ uint64_t* match_table(SegShared<RB, IPT, BLOCK>& sh, uint32_t wave) {
+    return sh.stage + wave * 64 * IPT;
+}
+
+// Zeroes the wave's match-mask table before a ranking by match masks (the
+// atomic-add ranking does not use it).  Orders the wave's LDS accesses.
+template <int RB, int IPT, int BLOCK>
+__device__ __forceinline__ void clear_match(const mgpuSegArgs& a, SegShared<RB, IPT, BLOCK>& sh,
+                                            uint32_t lane, uint32_t wave) {
+    if (a.rank1) return;
+    uint64_t* m = match_table(sh, wave);
+    for (uint32_t i = lane; i < (uint32_t)SegShared<RB, IPT, BLOCK>::RADIX; i += 64) m[i] = 0;
+    wave_sync();
+}
+
+// The payload a slot carries through the in-LDS passes: its value when the
+// sort has exactly one value array (read once, in order, at the start, and
+// written at the end with no gather), else its source index (the value
+// itself in index mode; the gather index of every array otherwise).
+__device__ __forceinline__ bool carries_value(const mgpuSegArgs& a) {
+    return !a.index_mode && a.num_values == 1;
+}
+
+// Writes the sorted keys and the values: the carried payload when it is the
+// value (one array, or indices), else every value array gathered through the
+// source indices, out[pos] = in[src[pos]].  `sync` orders the gathers of the
+// whole group before any store (the result side may be the input side).
 template <int IPT, typename Sync>
 __device__ __forceinline__ void emit(const mgpuSegArgs& a, const mgpuSegDesc& s, uint32_t n,
-                                     const uint32_t* key, const uint32_t* src, const uint32_t* idx,
+                                     const uint32_t* key, const uint32_t* pay, const uint32_t* idx,
                                      Sync sync) {
     const uint32_t side = a.result_side;
     const auto ko = mgpu::to_global(s.keys[side]);
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
         if (idx[j] < n) ko[idx[j]] = key[j];
-    if (a.index_mode) {
+    if (a.index_mode || carries_value(a)) {
         const auto vo = mgpu::to_global(s.vals[0][side]);
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) vo[idx[j]] = src[j];
+            if (idx[j] < n) vo[idx[j]] = pay[j];
         return;
     }
     for (uint32_t v = 0; v < a.num_values; ++v) {
@@ -115,12 +145,26 @@ __device__ __forceinline__ void emit(const mgpuSegArgs& a, const mgpuSegDesc& s,
         uint32_t val[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) val[j] = vi[src[j]];
+            if (idx[j] < n) val[j] = vi[pay[j]];
         sync();
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
             if (idx[j] < n) vo[idx[j]] = val[j];
         sync();
+    }
+}
+
+// Initial registers of an in-LDS sort: slot j's key and payload.
+template <int IPT>
+__device__ __forceinline__ void load_slots(const mgpuSegArgs& a, const mgpuSegDesc& s, uint32_t n,
+                                           const uint32_t* idx, uint32_t* key, uint32_t* pay) {
+    const auto ki = mgpu::to_global(s.keys[0]);
+    const bool value = carries_value(a);
+    const auto vi = mgpu::to_global(s.vals[0][0]);
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        key[j] = idx[j] < n ? ki[idx[j]] : 0u;
+        pay[j] = idx[j] < n && value ? vi[idx[j]] : idx[j];
     }
 }
 
@@ -131,18 +175,14 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
     constexpr int RADIX = SegShared<RB, IPT, BLOCK>::RADIX;
     constexpr int DPL = RADIX >= 64 ? RADIX / 64 : 1;  // digits per lane in the scan
     const uint32_t n = s.n;
-    uint32_t* stage = sh.stage + wave * 64 * IPT;
+    uint64_t* stage = sh.stage + wave * 64 * IPT;
     uint32_t* count = sh.count[wave];
-    uint64_t* match = sh.match[wave];
+    uint64_t* match = match_table(sh, wave);
 
-    uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
-    const auto ki = mgpu::to_global(s.keys[0]);
+    uint32_t key[IPT], pay[IPT], idx[IPT], pos[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        idx[j] = j * 64 + lane;
-        src[j] = idx[j];
-        key[j] = idx[j] < n ? ki[idx[j]] : 0u;
-    }
+    for (int j = 0; j < IPT; ++j) idx[j] = j * 64 + lane;
+    load_slots<IPT>(a, s, n, idx, key, pay);
     for (uint32_t p = 0; p < a.num_passes; ++p) {
         uint32_t shift, mask;
         pass_field(a, p, shift, mask);
@@ -150,6 +190,7 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
         for (int i = 0; i < DPL; ++i)
             if (lane * DPL + i < (uint32_t)RADIX) count[lane * DPL + i] = 0;
         wave_sync();
+        clear_match(a, sh, lane, wave);
         uint32_t rank[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
@@ -175,23 +216,19 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
         for (int j = 0; j < IPT; ++j)
             if (idx[j] < n) {
                 pos[j] = count[(key[j] >> shift) & mask] + rank[j];
-                stage[pos[j]] = key[j];
+                stage[pos[j]] = (uint64_t)pay[j] << 32 | key[j];
             }
         wave_sync();
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) key[j] = stage[idx[j]];
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) stage[pos[j]] = src[j];
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) src[j] = stage[idx[j]];
+            if (idx[j] < n) {
+                const uint64_t t = stage[idx[j]];
+                key[j] = (uint32_t)t;
+                pay[j] = (uint32_t)(t >> 32);
+            }
         wave_sync();
     }
-    emit<IPT>(a, s, n, key, src, idx, [] {
+    emit<IPT>(a, s, n, key, pay, idx, [] {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         wave_sync();
     });
@@ -204,25 +241,24 @@ __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared
     constexpr int RADIX = SegShared<RB, IPT, BLOCK>::RADIX;
     constexpr int W = BLOCK / 64;
     const uint32_t n = s.n;
-    uint32_t key[IPT], src[IPT], idx[IPT], pos[IPT];
-    const auto ki = mgpu::to_global(s.keys[0]);
+    uint32_t key[IPT], pay[IPT], idx[IPT], pos[IPT];
 #pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        idx[j] = wave * 64 * IPT + j * 64 + lane;
-        src[j] = idx[j];
-        key[j] = idx[j] < n ? ki[idx[j]] : 0u;
-    }
+    for (int j = 0; j < IPT; ++j) idx[j] = wave * 64 * IPT + j * 64 + lane;
+    load_slots<IPT>(a, s, n, idx, key, pay);
     for (uint32_t p = 0; p < a.num_passes; ++p) {
         uint32_t shift, mask;
         pass_field(a, p, shift, mask);
+        // (The previous pass's reads of the stage end before this barrier,
+        // and its writes of the next pass come after two more.)
         for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&sh.count[0][0])[i] = 0;
         __syncthreads();
+        clear_match(a, sh, lane, wave);
         uint32_t rank[IPT];
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
             if (idx[j] < n)
-                rank[j] = seg_rank((key[j] >> shift) & mask, lane, sh.count[wave], sh.match[wave],
-                                   a.rank1 != 0);
+                rank[j] = seg_rank((key[j] >> shift) & mask, lane, sh.count[wave],
+                                   match_table(sh, wave), a.rank1 != 0);
         __syncthreads();
         uint32_t total = 0;
         if (tid < (uint32_t)RADIX) {
@@ -243,23 +279,18 @@ __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared
         for (int j = 0; j < IPT; ++j)
             if (idx[j] < n) {
                 pos[j] = sh.count[wave][(key[j] >> shift) & mask] + rank[j];
-                sh.stage[pos[j]] = key[j];
+                sh.stage[pos[j]] = (uint64_t)pay[j] << 32 | key[j];
             }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) key[j] = sh.stage[idx[j]];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) sh.stage[pos[j]] = src[j];
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j)
-            if (idx[j] < n) src[j] = sh.stage[idx[j]];
-        __syncthreads();
+            if (idx[j] < n) {
+                const uint64_t t = sh.stage[idx[j]];
+                key[j] = (uint32_t)t;
+                pay[j] = (uint32_t)(t >> 32);
+            }
     }
-    emit<IPT>(a, s, n, key, src, idx, [] { __syncthreads(); });
+    emit<IPT>(a, s, n, key, pay, idx, [] { __syncthreads(); });
 }
 
 // ---- one workgroup, passes through global memory ---------------------------
@@ -267,6 +298,7 @@ template <int RB, int IPT, int BLOCK>
 __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT, BLOCK>& sh,
                             uint32_t tid, uint32_t lane, uint32_t wave) {
     constexpr int RADIX = SegShared<RB, IPT, BLOCK>::RADIX;
+    uint32_t* const st = reinterpret_cast<uint32_t*>(sh.stage);  // TILE words of it
     constexpr int W = BLOCK / 64;
     constexpr uint32_t TILE = SegShared<RB, IPT, BLOCK>::TILE;
     const uint32_t n = s.n;
@@ -302,6 +334,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
         for (uint32_t tb = 0; tb < n; tb += TILE) {
             for (uint32_t i = tid; i < (uint32_t)(W * RADIX); i += BLOCK) (&sh.count[0][0])[i] = 0;
             __syncthreads();
+            clear_match(a, sh, lane, wave);
             uint32_t key[IPT], rank[IPT], pos[IPT], dest[IPT];
             const uint32_t wb = tb + wave * 64 * IPT;
 #pragma unroll
@@ -310,7 +343,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
                 if (i < n) {
                     key[j] = ki[i];
                     rank[j] = seg_rank((key[j] >> shift) & mask, lane, sh.count[wave],
-                                       sh.match[wave], a.rank1 != 0);
+                                       match_table(sh, wave), a.rank1 != 0);
                 }
             }
             __syncthreads();
@@ -336,7 +369,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
                 const uint32_t i = wb + j * 64 + lane;
                 if (i < n) {
                     pos[j] = sh.count[wave][(key[j] >> shift) & mask] + rank[j];
-                    sh.stage[pos[j]] = key[j];
+                    st[pos[j]] = key[j];
                 }
             }
             __syncthreads();
@@ -346,7 +379,7 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
                 const uint32_t q = j * BLOCK + tid;
                 dest[j] = n;
                 if (q < valid) {
-                    const uint32_t k = sh.stage[q];
+                    const uint32_t k = st[q];
                     dest[j] = sh.dst[(k >> shift) & mask] + q;
                     if (dest[j] < n) ko[dest[j]] = k;
                     else mgpu::set_error(a.error_flag, 4u);
@@ -365,14 +398,14 @@ __device__ void global_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShare
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t i = wb + j * 64 + lane;
-                    if (i < n) sh.stage[pos[j]] = val[j];
+                    if (i < n) st[pos[j]] = val[j];
                 }
                 __syncthreads();
                 const auto vo = mgpu::to_global(s.vals[v][out]);
 #pragma unroll
                 for (int j = 0; j < IPT; ++j) {
                     const uint32_t q = j * BLOCK + tid;
-                    if (q < valid && dest[j] < n) vo[dest[j]] = sh.stage[q];
+                    if (q < valid && dest[j] < n) vo[dest[j]] = st[q];
                 }
             }
             __syncthreads();
@@ -408,8 +441,6 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
     constexpr int W = BLOCK / 64;
     __shared__ SegShared<RB, IPT, BLOCK> sh;
     const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (uint32_t i = tid; i < (uint32_t)(W * SegShared<RB, IPT, BLOCK>::RADIX); i += BLOCK)
-        (&sh.match[0][0])[i] = 0;
 
     // Workgroup phase: block and global segments, largest first, from a
     // shared cursor.  (Dealt round-robin like the small ones below, they

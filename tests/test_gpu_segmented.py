@@ -98,6 +98,34 @@ def test_size_classes_per_variant(engine, oracle, variant, wide_min):
             os.environ["MGPUSORT_SEG_WIDE_MIN"] = old_wide
 
 
+@pytest.mark.parametrize("wide_min", [None, "65537"])
+def test_size_classes_match_mask_ranking(oracle, wide_min):
+    """The persistent kernels' ranking by match masks (an engine made with
+    MGPUSORT_RANK=0), whose per-wave mask tables share LDS with the staging
+    of keys and payloads, over every size class; with MGPUSORT_SEG_WIDE_MIN
+    =65537 also the one-workgroup global routine.  One value array (carried
+    through the passes) and two (gathered at the end)."""
+    saved = os.environ.get("MGPUSORT_SEG_WIDE_MIN")
+    eng = make_engine({"MGPUSORT_RANK": "0"})
+    try:
+        if wide_min:
+            os.environ["MGPUSORT_SEG_WIDE_MIN"] = wide_min
+        starts = _segments(SIZES)
+        n = int(starts[-1])
+        keys = workloads.uniform_keys(n, seed=43)
+        keys[: n // 3] &= np.uint32(0x0F0F)  # ties
+        for vc in (1, 2):
+            vals = [workloads.uniform_keys(n, seed=50 + j) for j in range(vc)]
+            g = _run_segmented(eng, keys, vals, vc, starts)
+            _check_segments(oracle, keys, vals, vc, starts, g.keys(), g.values(), 0, 32)
+    finally:
+        eng.close()
+        if saved is None:
+            os.environ.pop("MGPUSORT_SEG_WIDE_MIN", None)
+        else:
+            os.environ["MGPUSORT_SEG_WIDE_MIN"] = saved
+
+
 @pytest.mark.parametrize("vc", [-1, 0, 2, 6])
 @pytest.mark.parametrize("fb,eb", [(0, 32), (0, 24), (7, 25), (16, 32), (0, 5)])
 def test_value_counts_and_bit_ranges(engine, oracle, vc, fb, eb):
