@@ -18,9 +18,11 @@
 // segment per wave at a time.  The last workgroup to finish puts the cursor
 // back to zero, so a captured graph of this launch can be replayed as is.
 //
-// In-LDS routines carry (key, source index) through the passes and gather
-// every value array once at the end; the global routine carries the values
-// through each pass.
+// In-LDS routines carry (key, payload) through the passes, one 64-bit LDS
+// word per slot: the payload is the value with one value array, else the
+// source index (the value in index mode; several arrays are gathered through
+// it once at the end).  The global routine carries the values through each
+// pass.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
