@@ -236,6 +236,42 @@ __device__ void wave_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<
     });
 }
 
+// ---- one wave, at most 64 keys: a bitonic network in registers -------------
+// Lane i holds key i.  The network orders 64-bit compare keys (the sort's bit
+// field above the lane index, all ones past n), so equal fields keep their
+// input order and the result is the stable sort by the field, as the passes
+// would leave it; each lane then takes its key and payload from the lane its
+// compare key came from.  No LDS and no atomics: two shuffles per step, 21
+// steps.
+__device__ void wave_bitonic64(const mgpuSegArgs& a, const mgpuSegDesc& s, uint32_t lane) {
+    const uint32_t n = s.n;
+    uint32_t idx[1] = {lane}, key[1], pay[1];
+    load_slots<1>(a, s, n, idx, key, pay);
+    const uint32_t field = a.num_bits >= 32 ? key[0] >> a.first_bit
+                                            : (key[0] >> a.first_bit) & ((1u << a.num_bits) - 1u);
+    uint64_t ck = lane < n ? ((uint64_t)field << 6) | lane : ~0ull;
+#pragma unroll
+    for (uint32_t k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t olo = (uint32_t)__shfl_xor((int)(uint32_t)ck, (int)j, 64);
+            const uint32_t ohi = (uint32_t)__shfl_xor((int)(uint32_t)(ck >> 32), (int)j, 64);
+            const uint64_t other = ((uint64_t)ohi << 32) | olo;
+            // Ascending blocks of k (all of them at k = 64); the lower lane
+            // of a pair keeps the smaller key in an ascending block.
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            if (keep_min ? other < ck : other > ck) ck = other;
+        }
+    }
+    const int src = (int)((uint32_t)ck & 63u);
+    uint32_t k2[1] = {(uint32_t)__shfl((int)key[0], src, 64)};
+    uint32_t p2[1] = {(uint32_t)__shfl((int)pay[0], src, 64)};
+    emit<1>(a, s, n, k2, p2, idx, [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        wave_sync();
+    });
+}
+
 // ---- one workgroup, whole segment in registers + LDS ----------------------
 template <int RB, int IPT, int BLOCK>
 __device__ void block_sort(const mgpuSegArgs& a, const mgpuSegDesc& s, SegShared<RB, IPT, BLOCK>& sh,
@@ -472,8 +508,11 @@ __device__ void segsort_body(const mgpuSegArgs& a) {
     // over the grid's waves.  (A shared cursor costs a device-wide atomic on
     // one word per segment: ~25 ns each, serialized, which bounded a batch
     // of 130K segments of up to 64 keys at 3.3 ms whatever the kernel.)
-    for (uint32_t item = blockIdx.x * W + wave; item < a.num_small; item += gridDim.x * W)
-        wave_sort<RB, IPT, BLOCK>(a, a.segs[a.small_list[item]], sh, lane, wave);
+    for (uint32_t item = blockIdx.x * W + wave; item < a.num_small; item += gridDim.x * W) {
+        const mgpuSegDesc& s = a.segs[a.small_list[item]];
+        if (s.n <= 64) wave_bitonic64(a, s, lane);
+        else wave_sort<RB, IPT, BLOCK>(a, s, sh, lane, wave);
+    }
 
     __syncthreads();
     if (tid == 0) {
