@@ -20,7 +20,9 @@ LIB_DIR = os.path.join(_HERE, "lib")
 
 
 def library_path():
-    return os.path.join(LIB_DIR, "libmgpusort.so")
+    """libmgpusort.so beside the code objects, or the build MGPUSORT_LIBRARY
+    names (a sanitizer build, say; the code objects still come from LIB_DIR)."""
+    return os.environ.get("MGPUSORT_LIBRARY") or os.path.join(LIB_DIR, "libmgpusort.so")
 
 
 def kernel_path():

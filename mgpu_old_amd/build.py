@@ -3,6 +3,9 @@
 Outputs go to mgpu_old_amd/lib/ (git-ignored, shipped with the tree):
   mgpusort_radix.hsaco   digit_count, digit_scan, rank_scatter kernels
   libmgpusort.so         the C ABI of include/mgpusort.h (host C++ over HIP)
+
+python -m mgpu_old_amd.build --ubsan DIR builds DIR/libmgpusort.so with the
+host code under UndefinedBehaviorSanitizer (load it with MGPUSORT_LIBRARY).
 """
 import os
 import subprocess
@@ -55,15 +58,41 @@ def build(verbose=True):
               "-I", CSRC, "-I", INCLUDE, os.path.join(CSRC, src), "-o", obj])
         objects.append(obj)
     # Host code only: plain g++ against the HIP runtime, no device pass.
-    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
+    _link_host(os.path.join(LIB, "libmgpusort.so"), objects, ["-O2"])
+
+
+def _link_host(out, objects, flags):
+    _run([CXX, *flags, "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra",
           "-fvisibility=hidden", "-D__HIP_PLATFORM_AMD__",
           "-I", CSRC, "-I", INCLUDE, "-I", os.path.join(ROCM, "include"),
           *[os.path.join(CSRC, s) for s in HOST_SOURCES], *objects,
-          "-o", os.path.join(LIB, "libmgpusort.so"),
+          "-o", out,
           "-L", os.path.join(ROCM, "lib"), f"-Wl,-rpath,{os.path.join(ROCM, 'lib')}",
           "-lamdhip64", "-ldl", "-pthread"])
 
 
+# Host code under UndefinedBehaviorSanitizer (aborting at the first report):
+# a shared library that needs no preloaded runtime, so any process that loads
+# it through MGPUSORT_LIBRARY checks the host side of whatever it runs (the
+# kernels are the same code objects).  Needs the normal build's noalloc.o.
+UBSAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=undefined",
+               "-fno-sanitize-recover=undefined"]
+
+
+def build_ubsan(out_dir):
+    os.makedirs(out_dir, exist_ok=True)
+    objects = [os.path.join(LIB, os.path.splitext(src)[0] + ".o") for src in HIP_HOST_SOURCES]
+    for o in objects:
+        if not os.path.exists(o):
+            raise FileNotFoundError(f"{o}: run the normal build first")
+    out = os.path.join(out_dir, "libmgpusort.so")
+    _link_host(out, objects, UBSAN_FLAGS)
+    return out
+
+
 if __name__ == "__main__":
-    build()
+    if len(sys.argv) == 3 and sys.argv[1] == "--ubsan":
+        build_ubsan(sys.argv[2])
+    else:
+        build()
     sys.exit(0)

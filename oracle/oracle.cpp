@@ -136,7 +136,7 @@ void sort2_pass(std::vector<uint32_t>& keys, std::vector<uint32_t>& perm,
 void gather_outputs(const uint32_t* const* values, int valueCount, size_t n,
                     const std::vector<uint32_t>& perm, uint32_t* const* valuesOut) {
     if (valueCount == -1) {
-        std::memcpy(valuesOut[0], perm.data(), n * sizeof(uint32_t));
+        if (n) std::memcpy(valuesOut[0], perm.data(), n * sizeof(uint32_t));
         return;
     }
     for (int v = 0; v < valueCount; ++v)
@@ -234,7 +234,7 @@ int oracle_sort1(const uint32_t* keys, const uint32_t* const* values, int valueC
         // One block covering the whole array is exactly Sort1Pass.
         sort2_pass(k, perm, bit, nb, n ? n : 1, 1);
     }
-    std::memcpy(keysOut, k.data(), n * sizeof(uint32_t));
+    if (n) std::memcpy(keysOut, k.data(), n * sizeof(uint32_t));
     gather_outputs(values, valueCount, n, perm, valuesOut);
     return 0;
 }
@@ -253,7 +253,7 @@ int oracle_sort2(const uint32_t* keys, const uint32_t* const* values, int valueC
         int nb = std::min(digitBits, endBit - bit);
         sort2_pass(k, perm, bit, nb, blockSize, threads);
     }
-    std::memcpy(keysOut, k.data(), n * sizeof(uint32_t));
+    if (n) std::memcpy(keysOut, k.data(), n * sizeof(uint32_t));
     gather_outputs(values, valueCount, n, perm, valuesOut);
     return 0;
 }
@@ -265,7 +265,7 @@ int oracle_block_counts(const uint32_t* keys, uint64_t n, int bit, int numBits,
     if (numBits < 1 || numBits > 16 || bit < 0 || bit + numBits > 32 || !blockSize) return -1;
     uint64_t numBuckets = 1ull << numBits;
     uint64_t numBlocks = (n + blockSize - 1) / blockSize;
-    std::memset(counts, 0, numBlocks * numBuckets * sizeof(uint32_t));
+    if (numBlocks) std::memset(counts, 0, numBlocks * numBuckets * sizeof(uint32_t));
     for (uint64_t i = 0; i < n; ++i)
         ++counts[(i / blockSize) * numBuckets + bitfield(keys[i], bit, numBits)];
     return 0;
