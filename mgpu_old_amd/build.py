@@ -71,23 +71,35 @@ def _link_host(out, objects, flags):
           "-lamdhip64", "-ldl", "-pthread"])
 
 
-# Host code under UndefinedBehaviorSanitizer (aborting at the first report):
-# a shared library that needs no preloaded runtime, so any process that loads
-# it through MGPUSORT_LIBRARY checks the host side of whatever it runs (the
-# kernels are the same code objects).  Needs the normal build's noalloc.o.
-UBSAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=undefined",
-               "-fno-sanitize-recover=undefined"]
+# Host code under sanitizers (the kernels are the same code objects); both
+# need the normal build's noalloc.o.
+#   ubsan: UndefinedBehaviorSanitizer, aborting at the first report: a shared
+#          library that needs no preloaded runtime, so any process that loads
+#          it through MGPUSORT_LIBRARY checks the host side of what it runs;
+#   asan:  AddressSanitizer + UBSan, for executables linked with
+#          -fsanitize=address (which load the ASan runtime first), such as the
+#          C++ client's tests/cpp/client_asan.
+SANITIZER_FLAGS = {
+    "ubsan": ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=undefined",
+              "-fno-sanitize-recover=undefined"],
+    "asan": ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=undefined"],
+}
 
 
-def build_ubsan(out_dir):
+def build_sanitized(out_dir, kind):
     os.makedirs(out_dir, exist_ok=True)
     objects = [os.path.join(LIB, os.path.splitext(src)[0] + ".o") for src in HIP_HOST_SOURCES]
     for o in objects:
         if not os.path.exists(o):
             raise FileNotFoundError(f"{o}: run the normal build first")
     out = os.path.join(out_dir, "libmgpusort.so")
-    _link_host(out, objects, UBSAN_FLAGS)
+    _link_host(out, objects, SANITIZER_FLAGS[kind])
     return out
+
+
+def build_ubsan(out_dir):
+    return build_sanitized(out_dir, "ubsan")
 
 
 if __name__ == "__main__":

@@ -173,6 +173,37 @@ def test_plan_replay(engine, oracle, eb):
     plan.close()
 
 
+def test_plan_replay_determinism(engine, oracle):
+    """A race check: one captured plan over a mixed batch (tiny segments on
+    the register network, wave and workgroup sorts, device-wide segments with
+    the count pieces and grouped tile counters), replayed 10 times from the
+    same input, gives the same bits every time, and the first result is the
+    oracle's."""
+    import torch
+    sizes = workloads.log_uniform_sizes(1500, 0, 16, seed=21)
+    starts = _segments(sizes)
+    n = int(starts[-1])
+    keys = workloads.and3_keys(n, seed=22)  # skewed digits: many equal ranks
+    vals = [np.arange(n, dtype=np.uint32)]
+    g = GpuSort(np.zeros(n, dtype=np.uint32), [np.zeros(n, dtype=np.uint32)])
+    plan = engine.segmented_plan(g.data, starts)
+    first = None
+    for rep in range(10):
+        g.by_ptr[g.data.keys[0]][:n].copy_(to_dev(keys)[:n])
+        g.by_ptr[g.data.values[0][0]][:n].copy_(to_dev(vals[0])[:n])
+        torch.cuda.synchronize()
+        plan.execute(g.data)
+        torch.cuda.synchronize()
+        engine.check_device()
+        out = (g.keys().copy(), g.values()[0].copy())
+        if first is None:
+            first = out
+            _check_segments(oracle, keys, vals, 1, starts, out[0], [out[1]], 0, 32)
+        else:
+            assert np.array_equal(out[0], first[0]) and np.array_equal(out[1], first[1]), rep
+    plan.close()
+
+
 def test_batch_of_separate_buffers_alone_vs_mix(engine, oracle):
     """sortSegmentedBatch over arrays with buffers of their own; each result
     equals sortArray on that array alone."""
