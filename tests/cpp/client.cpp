@@ -6,7 +6,11 @@
 //   2. bwt-style (bwt/src/mgpubwt/mgpubwt.cpp:262-292): multi-word keys
 //      sorted least significant word first by repeated sortArray calls that
 //      carry the remaining words plus an index stream as values, with the
-//      caller tracking buffers through `parity`.
+//      caller tracking buffers through `parity`;
+//   3. the additions with no reference counterpart: a segmented sort of 600
+//      arrays (sortSegmentedArray, then a captured plan replayed twice) and
+//      40 requests of mixed sizes, bit widths and value modes through the
+//      sort service.
 // Every result is checked against std::stable_sort on the host.  Prints
 // "CLIENT OK" on success and exits non-zero on the first mismatch.
 #include <hip/hip_runtime.h>
@@ -158,12 +162,168 @@ static int bwt_style_test(MgpuSort& engine) {
     return 0;
 }
 
+// Host check of one array: keys (and `vals`, gathered) stably sorted by the
+// bit field [0, endBit) of the keys; indices when index is true.
+static bool sorted_like_host(const std::vector<uint32_t>& keys,
+                             const std::vector<std::vector<uint32_t>>& vals, int endBit,
+                             bool index, const std::vector<uint32_t>& outKeys,
+                             const std::vector<std::vector<uint32_t>>& outVals) {
+    const size_t n = keys.size();
+    const uint32_t mask = endBit >= 32 ? 0xFFFFFFFFu : (1u << endBit) - 1u;
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](uint32_t a, uint32_t b) { return (keys[a] & mask) < (keys[b] & mask); });
+    for (size_t i = 0; i < n; ++i) {
+        if (outKeys[i] != keys[perm[i]]) return false;
+        if (index && outVals[0][i] != perm[i]) return false;
+        if (!index)
+            for (size_t v = 0; v < vals.size(); ++v)
+                if (outVals[v][i] != vals[v][perm[i]]) return false;
+    }
+    return true;
+}
+
+static int segmented_test(MgpuSort& engine) {
+    std::mt19937 rng(13);
+    const int segs = 600;
+    std::vector<int> starts(segs + 1, 0);
+    std::uniform_real_distribution<double> u(0.0, 16.0);
+    for (int i = 0; i < segs; ++i) starts[i + 1] = starts[i] + (int)std::pow(2.0, u(rng));
+    const int n = starts[segs];
+    std::vector<uint32_t> keys(n), vals(n);
+    for (int i = 0; i < n; ++i) {
+        keys[i] = rng();
+        vals[i] = (uint32_t)i * 3u;
+    }
+    uint32_t *k0 = dalloc(n), *k1 = dalloc(n), *v0 = dalloc(n), *v1 = dalloc(n);
+    CHECK(k0 && k1 && v0 && v1);
+    sortData_d d;
+    std::memset(&d, 0, sizeof(d));
+    d.maxElements = d.numElements = n;
+    d.valueCount = 1;
+    d.firstBit = 0;
+    d.endBit = 32;
+    d.pKeys[0] = k0;
+    d.pKeys[1] = k1;
+    d.pValues[0][0] = v0;
+    d.pValues[0][1] = v1;
+    auto upload = [&]() {
+        return hipMemcpy(d.pKeys[0], keys.data(), 4 * (size_t)n, hipMemcpyHostToDevice) ==
+                   hipSuccess &&
+               hipMemcpy(d.pValues[0][0], vals.data(), 4 * (size_t)n, hipMemcpyHostToDevice) ==
+                   hipSuccess;
+    };
+    auto check = [&]() {
+        std::vector<uint32_t> ok(n), ov(n);
+        if (hipMemcpy(ok.data(), d.pKeys[0], 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(ov.data(), d.pValues[0][0], 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess)
+            return false;
+        for (int s = 0; s < segs; ++s) {
+            const int a = starts[s], b = starts[s + 1];
+            std::vector<uint32_t> sk(keys.begin() + a, keys.begin() + b);
+            std::vector<std::vector<uint32_t>> sv{{vals.begin() + a, vals.begin() + b}};
+            std::vector<uint32_t> gk(ok.begin() + a, ok.begin() + b);
+            std::vector<std::vector<uint32_t>> gv{{ov.begin() + a, ov.begin() + b}};
+            if (!sorted_like_host(sk, sv, 32, false, gk, gv)) return false;
+        }
+        return true;
+    };
+    CHECK(upload());
+    CHECK(sortSegmentedArray(engine, &d, segs, starts.data()) == SORT_STATUS_SUCCESS);
+    CHECK(check());
+    sortSegPlan_t plan = nullptr;
+    CHECK(sortSegmentedPlanCreate(engine, &d, segs, starts.data(), &plan) == SORT_STATUS_SUCCESS);
+    for (int rep = 0; rep < 2; ++rep) {
+        CHECK(upload());
+        CHECK(sortSegmentedPlanExecute(plan, &d) == SORT_STATUS_SUCCESS);
+        CHECK(hipDeviceSynchronize() == hipSuccess);
+        CHECK(check());
+    }
+    CHECK(sortSegmentedPlanDestroy(plan) == SORT_STATUS_SUCCESS);
+    for (uint32_t* p : {k0, k1, v0, v1}) (void)hipFree(p);
+    return 0;
+}
+
+static int server_test(MgpuSort& engine) {
+    sortServer_t srv = nullptr;
+    CHECK(sortServerCreate(engine, 1ull << 30, 3, 1 << 20, &srv) == SORT_STATUS_SUCCESS);
+    std::mt19937 rng(17);
+    std::uniform_real_distribution<double> u(0.0, 18.0);
+    const int count = 40;
+    static const int kWidths[3] = {16, 24, 32}, kValueCounts[4] = {0, 1, 2, -1};
+    struct Req {
+        std::vector<uint32_t> keys;
+        std::vector<std::vector<uint32_t>> vals;
+        uint32_t* dk = nullptr;
+        uint32_t* dv[2] = {nullptr, nullptr};
+        int vc = 0, endBit = 32;
+    };
+    std::vector<Req> reqs(count);
+    for (int i = 0; i < count; ++i) {
+        Req& r = reqs[i];
+        const int n = (int)std::pow(2.0, u(rng));
+        r.endBit = kWidths[i % 3];
+        r.vc = kValueCounts[i % 4];
+        const uint32_t mask = r.endBit >= 32 ? 0xFFFFFFFFu : (1u << r.endBit) - 1u;
+        r.keys.resize(n);
+        for (int j = 0; j < n; ++j) r.keys[j] = rng() & mask;
+        CHECK((r.dk = dalloc(n)) != nullptr);
+        CHECK(hipMemcpy(r.dk, r.keys.data(), 4 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess);
+        const int arrays = r.vc == -1 ? 1 : r.vc;
+        for (int v = 0; v < arrays; ++v) {
+            CHECK((r.dv[v] = dalloc(n)) != nullptr);
+            if (r.vc > 0) {
+                r.vals.emplace_back(n);
+                for (int j = 0; j < n; ++j) r.vals.back()[j] = rng();
+                CHECK(hipMemcpy(r.dv[v], r.vals.back().data(), 4 * (size_t)n,
+                                hipMemcpyHostToDevice) == hipSuccess);
+            }
+        }
+        sortRequest_d q;
+        std::memset(&q, 0, sizeof(q));
+        q.keys = (CUdeviceptr)(uintptr_t)r.dk;
+        for (int v = 0; v < arrays; ++v) q.values[v] = (CUdeviceptr)(uintptr_t)r.dv[v];
+        q.numElements = n;
+        q.valueCount = r.vc;
+        q.firstBit = 0;
+        q.endBit = r.endBit;
+        q.tag = (unsigned long long)i;
+        CHECK(sortServerSubmit(srv, &q) == SORT_STATUS_SUCCESS);
+    }
+    int done = 0;
+    sortCompletion_d comps[64];
+    for (int polls = 0; done < count && polls < 120; ++polls) {
+        const int got = sortServerPoll(srv, comps, 64, 1000);
+        CHECK(got >= 0);
+        for (int c = 0; c < got; ++c) CHECK(comps[c].status == SORT_STATUS_SUCCESS);
+        done += got;
+    }
+    CHECK(done == count);
+    CHECK(sortServerDestroy(srv) == SORT_STATUS_SUCCESS);
+    for (Req& r : reqs) {
+        const size_t n = r.keys.size();
+        const int arrays = r.vc == -1 ? 1 : r.vc;
+        std::vector<uint32_t> ok(n);
+        std::vector<std::vector<uint32_t>> ov(arrays, std::vector<uint32_t>(n));
+        CHECK(hipMemcpy(ok.data(), r.dk, 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+        for (int v = 0; v < arrays; ++v)
+            CHECK(hipMemcpy(ov[v].data(), r.dv[v], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+        CHECK(sorted_like_host(r.keys, r.vals, r.endBit, r.vc == -1, ok, ov));
+        (void)hipFree(r.dk);
+        for (int v = 0; v < arrays; ++v) (void)hipFree(r.dv[v]);
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     const char* path = argc > 1 ? argv[1] : nullptr;
     try {
         MgpuSort engine(path);
         if (simple_test(engine)) return 1;
         if (bwt_style_test(engine)) return 1;
+        if (segmented_test(engine)) return 1;
+        if (server_test(engine)) return 1;
     } catch (const MgpuException& e) {
         std::printf("MgpuException: %s\n", e.what());
         return 2;
