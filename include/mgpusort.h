@@ -351,10 +351,22 @@ sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info)
  * by their bits at the ends.  No sortData_d field changes; the float mapping
  * is the reference's FloatToUint / UintToFloat (sort/src/kernels/common.cu:
  * 159-185), which the reference defines but never calls. */
+#define SORT_KEYS_UINT32 0
 #define SORT_KEYS_INT32 1
 #define SORT_KEYS_FLOAT32 2
 sortStatus_t SORTAPI sortTransformKeys(sortEngine_t engine, sortDevicePtr_t keys, int numElements,
                                        int keyType, bool toRadixOrder);
+
+/* sortArray over int32 or float32 keys (keyType SORT_KEYS_INT32 or
+ * SORT_KEYS_FLOAT32; SORT_KEYS_UINT32 is sortArray itself) in numeric order,
+ * without the two passes over the keys that sortTransformKeys adds: the
+ * count and the first pass map each key as they read it, and the last pass
+ * maps it back as it writes it.  Results, buffer swaps, parity, earlyExit
+ * and the bit range behave as sortTransformKeys, sortArray,
+ * sortTransformKeys would (firstBit/endBit select bits of the mapped key);
+ * keys come back in their own type.  No counterpart in the reference, which
+ * sorts unsigned keys only. */
+sortStatus_t SORTAPI sortArrayTyped(sortEngine_t engine, sortData_t data, int keyType);
 
 /* Per-kernel timing: while enabled, every launch is bracketed by hipEvents
  * on the engine's stream.  sortReadTiming waits for the stream, fills up to

@@ -49,7 +49,8 @@ class SortStatus:
 
 
 SORT_VALUE_INDEX = -1
-KEYS_INT32 = 1      # SORT_KEYS_INT32: sortTransformKeys key types
+KEYS_UINT32 = 0     # SORT_KEYS_UINT32: sortTransformKeys / sortArrayTyped key types
+KEYS_INT32 = 1      # SORT_KEYS_INT32
 KEYS_FLOAT32 = 2    # SORT_KEYS_FLOAT32
 SORT_MAX_ELEMENTS = (1 << 30) - 1
 
@@ -157,6 +158,7 @@ def lib():
             "sortAllocData": (ctypes.c_int, [eng, sd]),
             "sortFreeData": (ctypes.c_int, [sd]),
             "sortArray": (ctypes.c_int, [eng, sd]),
+            "sortArrayTyped": (ctypes.c_int, [eng, sd, ctypes.c_int]),
             "sortArrayEx": (ctypes.c_int, [eng, sd, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_bool]),
             "sortHost": (ctypes.c_int, [eng, P(ctypes.c_uint32), P(ctypes.c_uint32),
@@ -286,6 +288,12 @@ class Engine:
 
     def sort_array(self, data):
         _check(lib().sortArray(self._h, ctypes.byref(data)), "sortArray")
+
+    def sort_array_typed(self, data, key_type):
+        """sortArrayTyped: sortArray over int32 (KEYS_INT32) or float32
+        (KEYS_FLOAT32) keys in numeric order, the order mapping done inside
+        the passes."""
+        _check(lib().sortArrayTyped(self._h, ctypes.byref(data), int(key_type)), "sortArrayTyped")
 
     def sort_segmented(self, data, starts):
         """sortSegmentedArray: starts = numSegments + 1 offsets into data."""

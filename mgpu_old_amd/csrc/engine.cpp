@@ -286,6 +286,20 @@ sortStatus_t mgpu_launch(sortEngine_d* e, hipStream_t stream, hipFunction_t f, u
     return SORT_STATUS_SUCCESS;
 }
 
+// key_transform of n keys in place on `stream` (type MGPU_KEYS_*).
+static sortStatus_t mgpu_key_transform_launch(sortEngine_d* e, hipStream_t stream,
+                                              uint32_t* keys, uint32_t n, uint32_t type,
+                                              bool toRadix) {
+    mgpuKeyTransformArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.keys = keys;
+    a.n = n;
+    a.type = type;
+    a.to_radix = toRadix ? 1u : 0u;
+    const uint32_t grid = std::min<uint32_t>((n + 1023) / 1024, (uint32_t)e->numCUs * 8);
+    return mgpu_launch(e, stream, e->fnKeyTransform, grid, 256, &a);
+}
+
 // One sort of n keys by [firstBit, firstBit + sum(widths)) from in-buffers to
 // out-buffers alternating per pass: pass p reads keys[p&1]-side pointers.
 // Pointers are passed per pass by the caller through `bufs`.
@@ -293,11 +307,24 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
                               int numPasses, const int* widths, int valueCount,
                               uint32_t* const keyBuf[2],
                               uint32_t* const valBuf[MGPU_MAX_VALUES][2], bool earlyExit,
-                              int* executed) {
+                              int* executed, int keyType) {
     int done = 0;
     if (executed) *executed = 0;
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     if (numPasses > MGPU_MAX_PASSES) return SORT_STATUS_INTERNAL_ERROR;
+    // Signed and float keys: the kernels' key type, and the count's form of
+    // the mapping to the unsigned order (mgpu_key_map's two words).
+    uint32_t kernelKeyType = 0, toLo = 0, toHi = 0;
+    if (keyType == SORT_KEYS_INT32) {
+        kernelKeyType = MGPU_KEYS_INT32;
+        toLo = toHi = 0x80000000u;
+    } else if (keyType == SORT_KEYS_FLOAT32) {
+        kernelKeyType = MGPU_KEYS_FLOAT32;
+        toLo = 0x80000000u;
+        toHi = 0xFFFFFFFFu;
+    } else if (keyType != SORT_KEYS_UINT32) {
+        return SORT_STATUS_INVALID_VALUE;
+    }
     const SortPlanChoice choice = mgpu_pick_plan(e, n, valueCount);
     const PassVariant& var = *choice.variant;
     const uint32_t tile = var.tile();
@@ -305,6 +332,18 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
     const uint64_t lbWords = (uint64_t)numPasses * numTiles * MGPU_RADIX;
     sortStatus_t st = SORT_STATUS_SUCCESS;
     const bool timed = !L.noAlloc;
+    // Typed keys are mapped inside the count and the first and last executed
+    // passes where the tile shape has those forms (atomic-add ranking), else
+    // in place by key_transform before the count and after the last pass.
+    const uint32_t typedKeys = kernelKeyType;
+    const bool fused = typedKeys != 0 && var.passKM[0] != nullptr && var.xsegKM[0] != nullptr &&
+                       e->fnCountKM != nullptr && e->fnCountColsKM != nullptr;
+    if (typedKeys != 0 && !fused) {
+        if (L.noAlloc) return SORT_STATUS_CONFIG_NOT_SUPPORTED;
+        st = mgpu_key_transform_launch(e, L.stream, keyBuf[0], n, typedKeys, true);
+        if (st) return st;
+        kernelKeyType = 0;
+    }
     if (L.noAlloc) {
         // Scratch of a fixed size, filled by nothing that allocates or waits.
         if (lbWords > L.lookbackWords) return SORT_STATUS_CONFIG_NOT_SUPPORTED;
@@ -354,6 +393,10 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         ca.check_shift = (uint32_t)firstBit;
         ca.check_mask = totalBits >= 32 ? 0xffffffffu : ((1u << totalBits) - 1u);
     }
+    if (fused) {
+        ca.map_lo = toLo;
+        ca.map_hi = toHi;
+    }
     // Chunks cut per segment: of about 8192 keys for the 256-thread kernel (at
     // most 8 workgroups per CU), 32768 for the lane-column one (one per CU).
     ca.num_segs = segs;
@@ -386,8 +429,9 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         grid = ca.chunks_per_seg * segs;
     }
     L.histDirty = true;
-    st = mgpu_launch(e, L.stream, cols ? e->fnCountCols : e->fnCount, grid, countThreads, &ca,
-                     timed);
+    hipFunction_t countFn = cols ? (fused ? e->fnCountColsKM : e->fnCountCols)
+                                 : (fused ? e->fnCountKM : e->fnCount);
+    st = mgpu_launch(e, L.stream, countFn, grid, countThreads, &ca, timed);
     if (st) return st;
 
     // digit_scan (also puts hist and the counters back to zero; with early
@@ -442,8 +486,13 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         }
     }
     L.histDirty = false;
+    int lastRun = -1;
+    for (int p = 0; p < numPasses; ++p)
+        if (run[p]) lastRun = p;
 
-    // rank_scatter per executed pass; executed pass k reads side k & 1.
+    // rank_scatter per executed pass; executed pass k reads side k & 1.  The
+    // first maps signed or float keys to their unsigned order as it reads
+    // them, the last maps them back as it writes them.
     int bit = firstBit;
     for (int p = 0; p < numPasses; ++p) {
         const int shiftBits = bit;
@@ -470,6 +519,9 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
         pa.finished = L.tileCounters + MGPU_FINISHED_SLOT + p;
         pa.error_flag = L.errorDev;
         pa.spin_limit = e->spinLimit;
+        const bool maps = fused && (done == 0 || p == lastRun);
+        if (maps && done == 0) pa.key_types |= kernelKeyType;
+        if (maps && p == lastRun) pa.key_types |= kernelKeyType << 8;
         pa.n = n;
         pa.num_tiles = numTiles;
         pa.shift = (uint32_t)shiftBits;
@@ -491,15 +543,23 @@ sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& L, uint32_t n, int firs
             // Segment workgroups loop over their XCD's tiles: as many as fit
             // at once (at least one per XCD).
             const uint32_t g = std::max(segs, std::min(numTiles, var.xsegGrid));
-            hipFunction_t f = p == eePass && var.xsegEE[mode] ? var.xsegEE[mode] : var.xseg[mode];
+            hipFunction_t f = maps                              ? var.xsegKM[mode]
+                              : p == eePass && var.xsegEE[mode] ? var.xsegEE[mode]
+                                                                : var.xseg[mode];
             st = mgpu_launch(e, L.stream, f, g, var.blockThreads, &pa, timed);
         } else {
-            hipFunction_t f = p == eePass && var.passEE[mode] ? var.passEE[mode] : var.pass[mode];
+            hipFunction_t f = maps                              ? var.passKM[mode]
+                              : p == eePass && var.passEE[mode] ? var.passEE[mode]
+                                                                : var.pass[mode];
             st = mgpu_launch(e, L.stream, f, numTiles, var.blockThreads, &pa, timed);
         }
         if (st) return st;
         ++done;
         if (executed) *executed = done;
+    }
+    if (typedKeys != 0 && !fused) {
+        st = mgpu_key_transform_launch(e, L.stream, keyBuf[done & 1], n, typedKeys, false);
+        if (st) return st;
     }
     // Index generation with no pass left to run: the indices are the identity.
     if (done == 0 && valueCount == -1) {
@@ -531,7 +591,7 @@ sortStatus_t mgpu_validate_data(sortEngine_d* engine, const sortData_d* data) {
 // runs the passes, swaps every buffer pair once per executed pass and toggles
 // parity, exactly as sortPass does (mgpusort.cpp:518-526).
 sortStatus_t mgpu_sort_data(sortEngine_d* e, SortLane& lane, sortData_d* data, int numPasses,
-                            const int* widths) {
+                            const int* widths, int keyType) {
     const uint32_t n = (uint32_t)data->numElements;
     if (n == 0 || numPasses == 0) return SORT_STATUS_SUCCESS;
     const int carried = data->valueCount == -1 ? 1 : data->valueCount;
@@ -552,7 +612,7 @@ sortStatus_t mgpu_sort_data(sortEngine_d* e, SortLane& lane, sortData_d* data, i
     int executed = 0;
     sortStatus_t st = mgpu_sort_passes(e, lane, n, data->firstBit, numPasses, widths,
                                        data->valueCount, keyBuf, valBuf, data->earlyExit,
-                                       &executed);
+                                       &executed, keyType);
     if (st) return st;
     if (executed & 1) {
         std::swap(data->keys[0], data->keys[1]);
@@ -675,15 +735,9 @@ sortStatus_t SORTAPI sortTransformKeys(sortEngine_t engine, sortDevicePtr_t keys
         (keyType != SORT_KEYS_INT32 && keyType != SORT_KEYS_FLOAT32) || (!keys && numElements))
         return SORT_STATUS_INVALID_VALUE;
     if (numElements == 0) return SORT_STATUS_SUCCESS;
-    mgpuKeyTransformArgs a;
-    std::memset(&a, 0, sizeof(a));
-    a.keys = reinterpret_cast<uint32_t*>(keys);
-    a.n = (uint32_t)numElements;
-    a.type = keyType == SORT_KEYS_INT32 ? MGPU_KEYS_INT32 : MGPU_KEYS_FLOAT32;
-    a.to_radix = toRadixOrder ? 1u : 0u;
-    const uint32_t grid =
-        std::min<uint32_t>(((uint32_t)numElements + 1023) / 1024, (uint32_t)engine->numCUs * 8);
-    return mgpu_launch(engine, engine->lane.stream, engine->fnKeyTransform, grid, 256, &a);
+    return mgpu_key_transform_launch(
+        engine, engine->lane.stream, reinterpret_cast<uint32_t*>(keys), (uint32_t)numElements,
+        keyType == SORT_KEYS_INT32 ? MGPU_KEYS_INT32 : MGPU_KEYS_FLOAT32, toRadixOrder);
 }
 
 sortStatus_t SORTAPI sortEngineInfo(sortEngine_t engine, sortEngineInfo_t* info) {
@@ -756,6 +810,23 @@ sortStatus_t SORTAPI sortArray(sortEngine_t engine, sortData_t data) {
     int widths[MGPU_MAX_PASSES];
     int passes = mgpu_plan_passes(numBits, MGPU_RADIX_BITS, widths);
     return sort_with_widths(engine, data, passes, widths);
+}
+
+// sortArray over int32 or float32 keys, with the order mapping inside the
+// count and the first and last executed passes (no pass of its own).
+sortStatus_t SORTAPI sortArrayTyped(sortEngine_t engine, sortData_t data, int keyType) {
+    if (!engine || !data) return SORT_STATUS_INVALID_VALUE;
+    if (keyType != SORT_KEYS_UINT32 && keyType != SORT_KEYS_INT32 &&
+        keyType != SORT_KEYS_FLOAT32)
+        return SORT_STATUS_INVALID_VALUE;
+    if (mgpu_lane_take_error(engine->lane)) return SORT_STATUS_KERNEL_ERROR;
+    int numBits = data->endBit - data->firstBit;
+    if (!numBits) return SORT_STATUS_SUCCESS;
+    sortStatus_t st = mgpu_validate_data(engine, data);
+    if (st) return st;
+    int widths[MGPU_MAX_PASSES];
+    int passes = mgpu_plan_passes(numBits, MGPU_RADIX_BITS, widths);
+    return mgpu_sort_data(engine, engine->lane, data, passes, widths, keyType);
 }
 
 // sortArrayEx (mgpusort.cpp:618-648): the caller's digit width sets the pass

@@ -31,6 +31,11 @@ struct PassVariant {
     // shape): mgpu_rank_scatter_b*_i*_r1_ee_v<mode> and ..._r1_x_ee_v<mode>.
     hipFunction_t passEE[4] = {};
     hipFunction_t xsegEE[4] = {};
+    // Forms that map signed or float keys in a typed sort's first and last
+    // passes (atomic-add ranking only; null when not built for the shape):
+    // mgpu_rank_scatter_b*_i*_r1_km_v<mode> and ..._r1_x_km_v<mode>.
+    hipFunction_t passKM[4] = {};
+    hipFunction_t xsegKM[4] = {};
     uint32_t xsegGrid = 0;        // workgroups of an xseg launch: all that fit on the device
     uint32_t tile() const { return blockThreads * itemsPerThread; }
 };
@@ -109,6 +114,8 @@ struct sortEngine_d {
     hipModule_t radixModule = nullptr;
     hipFunction_t fnCount = nullptr;      // 256 threads, per-wave histograms
     hipFunction_t fnCountCols = nullptr;  // 1024 threads, lane-column histograms (large sorts)
+    hipFunction_t fnCountKM = nullptr;    // the two over signed or float keys
+    hipFunction_t fnCountColsKM = nullptr;
     hipFunction_t fnScan = nullptr;
     hipFunction_t fnIota = nullptr;
     hipFunction_t fnKeyTransform = nullptr;
@@ -226,12 +233,15 @@ sortStatus_t mgpu_ensure_multi(SortLane& lane, uint32_t arrays);
 // so the result is on side (executed & 1).  Without earlyExit every pass
 // runs; with it, passes whose digit is constant are skipped, and all of them
 // when the keys are already in order (one blocking readback).  valueCount as
-// in sortData_d.
+// in sortData_d.  keyType SORT_KEYS_INT32 / SORT_KEYS_FLOAT32: the keys are
+// read and checked in their unsigned order (sortTransformKeys' mapping) by
+// digit_count and the first executed pass, and mapped back by the last one.
 sortStatus_t mgpu_sort_passes(sortEngine_d* e, SortLane& lane, uint32_t n, int firstBit,
                               int numPasses, const int* widths, int valueCount,
                               uint32_t* const keyBuf[2],
                               uint32_t* const valBuf[MGPU_MAX_VALUES][2],
-                              bool earlyExit = false, int* executed = nullptr);
+                              bool earlyExit = false, int* executed = nullptr,
+                              int keyType = SORT_KEYS_UINT32);
 
 // Argument checks of sortArray / sortArrayEx (mgpusort.cpp:360-364,609-611).
 sortStatus_t mgpu_validate_data(sortEngine_d* e, const sortData_d* data);
@@ -239,4 +249,4 @@ sortStatus_t mgpu_validate_data(sortEngine_d* e, const sortData_d* data);
 // data's buffers, swaps every buffer pair once per executed pass and toggles
 // parity (mgpusort.cpp:518-526).
 sortStatus_t mgpu_sort_data(sortEngine_d* e, SortLane& lane, sortData_d* data, int numPasses,
-                            const int* widths);
+                            const int* widths, int keyType = SORT_KEYS_UINT32);

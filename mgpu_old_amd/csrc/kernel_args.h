@@ -86,6 +86,10 @@ struct mgpuCountArgs {
     uint32_t* unsorted;      // null: no check
     uint32_t check_shift;
     uint32_t check_mask;
+    // Signed or float keys are counted (and checked) as mgpu_key_map(k,
+    // map_lo, map_hi), their unsigned order; (0, 0) counts the keys as stored.
+    uint32_t map_lo;
+    uint32_t map_hi;
 };
 
 // digit_scan: per pass, where each (segment, digit) run starts in the output
@@ -150,6 +154,16 @@ static inline MGPU_HOST_DEVICE uint32_t mgpu_key_to_radix(uint32_t k, uint32_t t
 static inline MGPU_HOST_DEVICE uint32_t mgpu_key_from_radix(uint32_t u, uint32_t type) {
     if (type == MGPU_KEYS_INT32) return u ^ 0x80000000u;
     return (u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u;
+}
+
+// Both bijections (and the identity) in one form, for kernels that take the
+// direction and type as two words: k ^ (top bit of k set ? hi : lo).
+//   int32, either way:      lo = hi = 0x80000000
+//   float32 to the order:   lo = 0x80000000, hi = 0xFFFFFFFF
+//   float32 back:           lo = 0xFFFFFFFF, hi = 0x80000000
+//   unsigned (identity):    lo = hi = 0
+static inline MGPU_HOST_DEVICE uint32_t mgpu_key_map(uint32_t k, uint32_t lo, uint32_t hi) {
+    return k ^ ((k & 0x80000000u) ? hi : lo);
 }
 
 // ---------------------------------------------------------------------------
@@ -306,7 +320,12 @@ struct mgpuPassArgs {
     // *error_flag = 1 (0: every lookback gives up at once, which only tests
     // of the error path ask for).
     uint32_t spin_limit;
-    uint32_t pad_;
+    // Signed or float keys (MGPU_KEYS_INT32 / MGPU_KEYS_FLOAT32; 0: unsigned,
+    // left as they are): bits 0-7, the type of the keys the pass reads, which
+    // it maps to their unsigned order (the sort's first pass); bits 8-15, the
+    // type the keys it writes go back to (its last pass, which counts nothing
+    // for a next one).  One word: the kernels run short of scalar registers.
+    uint32_t key_types;
 };
 
 // Segments per XCD with counting for the next pass: at most this many.

@@ -62,6 +62,23 @@ sortStatus_t load_variant(sortEngine_d* e, uint32_t threads, uint32_t items, uin
             e->kernelNames[f] = name;
         }
     }
+    // Key-mapping forms of a typed sort, likewise (both, or neither).
+    for (int m = 0; m < 4 && rank == 1; ++m) {
+        char name[96], namex[96];
+        hipFunction_t f = nullptr, fx = nullptr;
+        std::snprintf(name, sizeof(name), "mgpu_rank_scatter_b%u_i%u_r1_km_v%d", threads, items, m);
+        std::snprintf(namex, sizeof(namex), "mgpu_rank_scatter_b%u_i%u_r1_x_km_v%d", threads,
+                      items, m);
+        if (hipModuleGetFunction(&f, e->radixModule, name) != hipSuccess ||
+            hipModuleGetFunction(&fx, e->radixModule, namex) != hipSuccess) {
+            for (int k = 0; k < 4; ++k) v->passKM[k] = v->xsegKM[k] = nullptr;
+            break;
+        }
+        v->passKM[m] = f;
+        v->xsegKM[m] = fx;
+        e->kernelNames[f] = name;
+        e->kernelNames[fx] = namex;
+    }
     // Packed multi-array forms, likewise.
     bool packed = rank == 1;
     for (int m = 0; m < 4 && packed; ++m) {
@@ -187,6 +204,13 @@ sortStatus_t mgpu_engine_load(sortEngine_d* e) {
     }
     sortStatus_t st = get_fn(e, e->radixModule, "mgpu_digit_count", &e->fnCount);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_count_cols", &e->fnCountCols);
+    // Counts of typed keys: without them (an older code object) typed sorts
+    // map the keys in place around the sort.
+    if (!st && (get_fn(e, e->radixModule, "mgpu_digit_count_km", &e->fnCountKM) ||
+                get_fn(e, e->radixModule, "mgpu_digit_count_cols_km", &e->fnCountColsKM))) {
+        e->fnCountKM = e->fnCountColsKM = nullptr;
+        (void)hipGetLastError();
+    }
     if (!st) st = get_fn(e, e->radixModule, "mgpu_digit_scan", &e->fnScan);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_iota", &e->fnIota);
     if (!st) st = get_fn(e, e->radixModule, "mgpu_key_transform", &e->fnKeyTransform);

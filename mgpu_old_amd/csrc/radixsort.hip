@@ -192,7 +192,7 @@ __device__ __forceinline__ void count1(uint32_t* h, uint32_t* spare, uint32_t i,
 // atomic add per key costs ~11 CU cycles per wave on random banks, ~5.6 on
 // distinct ones, ~54 when a third of the lanes share one word).  128 KB of
 // LDS: 1024 threads, one workgroup per CU.
-template <int BLOCK, int COLS>
+template <int BLOCK, int COLS, bool KMAP>
 __device__ void digit_count_body(const mgpuCountArgs& a) {
     constexpr int W = BLOCK / 64;
     constexpr uint32_t WORDS = COLS ? MGPU_MAX_PASSES * RADIX * COLS : W * MGPU_MAX_PASSES * RADIX;
@@ -253,6 +253,11 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
     // Fields are compared in place: x & M > y & M orders two keys as their
     // extracted fields would (both sit at the same bit position).
     const bool check = a.unsorted != nullptr;
+    // Signed or float keys (KMAP) are counted and checked in their unsigned
+    // order.
+    auto key_at = [&](uint32_t i) {
+        return KMAP ? mgpu_key_map(a.keys[i], a.map_lo, a.map_hi) : a.keys[i];
+    };
     const uint32_t mfull = a.check_mask << a.check_shift, mfirst = a.mask[0] << a.shift[0];
     uint32_t descents = 0, first_desc = 0;  // per lane (scalar tails)
     uint32_t wdesc = 0, wfirst = 0;         // per wave, by ballot (vector body)
@@ -287,7 +292,16 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                 }
                 const uint32_t ilast = i0 + (UNROLL - 1) * 64;
                 uint32_t edge = 0;
-                if (check && lane == 63 && ilast + 1 < nvec) edge = a.keys[begin + 4 * ilast + 4];
+                if (check && lane == 63 && ilast + 1 < nvec) edge = key_at(begin + 4 * ilast + 4);
+                if constexpr (KMAP) {
+#pragma unroll
+                    for (int u = 0; u < UNROLL; ++u) {
+                        q[u].x = mgpu_key_map(q[u].x, a.map_lo, a.map_hi);
+                        q[u].y = mgpu_key_map(q[u].y, a.map_lo, a.map_hi);
+                        q[u].z = mgpu_key_map(q[u].z, a.map_lo, a.map_hi);
+                        q[u].w = mgpu_key_map(q[u].w, a.map_lo, a.map_hi);
+                    }
+                }
 #pragma unroll
                 for (int u = 0; u < UNROLL; ++u) {
                     const uint32_t i = i0 + u * 64;
@@ -327,17 +341,17 @@ __device__ void digit_count_body(const mgpuCountArgs& a) {
                 }
             }
             if (check && tid == 0 && nvec > 0 && begin + 4 * nvec < a.n)
-                pair(a.keys[begin + 4 * nvec - 1], a.keys[begin + 4 * nvec]);
+                pair(key_at(begin + 4 * nvec - 1), key_at(begin + 4 * nvec));
             for (uint32_t i = begin + (nvec << 2) + tid; i < end; i += BLOCK) {
-                const uint32_t k = a.keys[i];
+                const uint32_t k = key_at(i);
                 count_key(k);
-                if (check && i + 1 < a.n) pair(k, a.keys[i + 1]);
+                if (check && i + 1 < a.n) pair(k, key_at(i + 1));
             }
         } else {
             for (uint32_t i = begin + tid; i < end; i += BLOCK) {
-                const uint32_t k = a.keys[i];
+                const uint32_t k = key_at(i);
                 count_key(k);
-                if (check && i + 1 < a.n) pair(k, a.keys[i + 1]);
+                if (check && i + 1 < a.n) pair(k, key_at(i + 1));
             }
         }
     }
@@ -879,7 +893,8 @@ __device__ __forceinline__ void load_striped(const uint32_t* src, uint32_t n, ui
 // per wave instead of per block in a count kernel).  Its keys then stage to
 // consecutive LDS words, free of bank conflicts; a tile whose waves all are in
 // order, and in order across waves, stages every key to its own place.
-template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool XSEG, bool EE, typename Args>
+template <int BLOCK, int IPT, int RANK, int VMODE, bool MULTI, bool XSEG, bool EE, bool KMAP,
+          typename Args>
 __device__ void rank_scatter_body(const Args& a) {
     static_assert(!EE || RANK == 1, "early exit builds on the atomic-add ranking");
     static_assert(!(EE && MULTI), "early exit is for single arrays");
@@ -1020,6 +1035,15 @@ __device__ void rank_scatter_body(const Args& a) {
         load_striped<IPT>(job.keys_in, n, wave_base, lane, key);
         if constexpr (VMODE == MGPU_VALUES_SINGLE)
             load_striped<IPT>(job_vals_in<MULTI>(a, job, 0), n, wave_base, lane, val);
+        // Signed or float keys (KMAP), in the sort's first pass: to their
+        // unsigned order.
+        if constexpr (KMAP) {
+            const uint32_t tin = a.key_types & 0xFFu;
+            if (tin != 0) {
+#pragma unroll
+                for (int j = 0; j < IPT; ++j) key[j] = mgpu_key_to_radix(key[j], tin);
+            }
+        }
 
         // Keys past n (last tile only) get every bit set: they take the pass's
         // top digit (mask) and, last in input order, rank after every real key
@@ -1190,6 +1214,15 @@ __device__ void rank_scatter_body(const Args& a) {
         for (int i = 0; i < IPT; ++i) {
             dest[i] = s_global[__builtin_amdgcn_ubfe(kout[i], shift, width)] + i * BLOCK + tid;
             if constexpr (JOINT) vout[i] = s_vstage[i * BLOCK + tid];
+        }
+        // Signed or float keys (KMAP), in the sort's last pass: back to their
+        // type (no pass after it reads their digits).
+        if constexpr (KMAP) {
+            const uint32_t tout = a.key_types >> 8;
+            if (tout != 0) {
+#pragma unroll
+                for (int i = 0; i < IPT; ++i) kout[i] = mgpu_key_from_radix(kout[i], tout);
+            }
         }
         // Offsets outside [0, n) can only come from a broken count or
         // lookback: they are flagged instead of written.  Slots with nothing
@@ -1643,11 +1676,20 @@ __device__ void packed_scatter_body(const mgpuMultiPassArgs& a) {
 // Exported entry points (names looked up by the engine).
 
 extern "C" __global__ void __launch_bounds__(256) mgpu_digit_count(mgpuCountArgs a) {
-    digit_count_body<256, 0>(a);
+    digit_count_body<256, 0, false>(a);
 }
 
 extern "C" __global__ void __launch_bounds__(1024) mgpu_digit_count_cols(mgpuCountArgs a) {
-    digit_count_body<1024, 32>(a);
+    digit_count_body<1024, 32, false>(a);
+}
+
+// The same over signed or float keys, counted in their unsigned order.
+extern "C" __global__ void __launch_bounds__(256) mgpu_digit_count_km(mgpuCountArgs a) {
+    digit_count_body<256, 0, true>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(1024) mgpu_digit_count_cols_km(mgpuCountArgs a) {
+    digit_count_body<1024, 32, true>(a);
 }
 
 extern "C" __global__ void __launch_bounds__(256) mgpu_digit_scan(mgpuScanArgs a) {
@@ -1697,30 +1739,47 @@ extern "C" __global__ void __launch_bounds__(256) mgpu_multi_scan(mgpuMultiScanA
 #define MGPU_RANK_SCATTER_MODE(B, I, R, V, MODE)                                             \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_v##V(mgpuPassArgs a) {                         \
-        rank_scatter_body<B, I, R, MODE, false, false, false>(a);                             \
+        rank_scatter_body<B, I, R, MODE, false, false, false, false>(a);                      \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r##R##_x_v##V(mgpuPassArgs a) {                       \
-        rank_scatter_body<B, I, R, MODE, false, true, false>(a);                              \
+        rank_scatter_body<B, I, R, MODE, false, true, false, false>(a);                       \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_multi_scatter_b##B##_i##I##_r##R##_v##V(mgpuMultiPassArgs a) {                   \
-        rank_scatter_body<B, I, R, MODE, true, false, false>(a);                              \
+        rank_scatter_body<B, I, R, MODE, true, false, false, false>(a);                       \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_multi_scatter_b##B##_i##I##_r##R##_x_v##V(mgpuMultiPassArgs a) {                 \
-        rank_scatter_body<B, I, R, MODE, true, true, false>(a);                               \
+        rank_scatter_body<B, I, R, MODE, true, true, false, false>(a);                        \
     }
 // Early-exit forms (atomic-add ranking only): one chain, and a chain per XCD.
 #define MGPU_RANK_SCATTER_EE_MODE(B, I, V, MODE)                                              \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r1_ee_v##V(mgpuPassArgs a) {                          \
-        rank_scatter_body<B, I, 1, MODE, false, false, true>(a);                              \
+        rank_scatter_body<B, I, 1, MODE, false, false, true, false>(a);                       \
     }                                                                                         \
     extern "C" __global__ void __launch_bounds__(B)                                           \
         mgpu_rank_scatter_b##B##_i##I##_r1_x_ee_v##V(mgpuPassArgs a) {                        \
-        rank_scatter_body<B, I, 1, MODE, false, true, true>(a);                               \
+        rank_scatter_body<B, I, 1, MODE, false, true, true, false>(a);                        \
     }
+// Forms that map signed or float keys to their unsigned order as they read
+// them, or back as they write them (atomic-add ranking only): the first and
+// last passes of a typed sort.
+#define MGPU_RANK_SCATTER_KM_MODE(B, I, V, MODE)                                              \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r1_km_v##V(mgpuPassArgs a) {                          \
+        rank_scatter_body<B, I, 1, MODE, false, false, false, true>(a);                       \
+    }                                                                                         \
+    extern "C" __global__ void __launch_bounds__(B)                                           \
+        mgpu_rank_scatter_b##B##_i##I##_r1_x_km_v##V(mgpuPassArgs a) {                        \
+        rank_scatter_body<B, I, 1, MODE, false, true, false, true>(a);                        \
+    }
+#define MGPU_RANK_SCATTER_KM(B, I)                                                            \
+    MGPU_RANK_SCATTER_KM_MODE(B, I, 0, MGPU_VALUES_NONE)                                      \
+    MGPU_RANK_SCATTER_KM_MODE(B, I, 1, MGPU_VALUES_INDEX)                                     \
+    MGPU_RANK_SCATTER_KM_MODE(B, I, 2, MGPU_VALUES_SINGLE)                                    \
+    MGPU_RANK_SCATTER_KM_MODE(B, I, 3, MGPU_VALUES_MULTI)
 #define MGPU_RANK_SCATTER_EE(B, I)                                                            \
     MGPU_RANK_SCATTER_EE_MODE(B, I, 0, MGPU_VALUES_NONE)                                      \
     MGPU_RANK_SCATTER_EE_MODE(B, I, 1, MGPU_VALUES_INDEX)                                     \
@@ -1761,6 +1820,8 @@ MGPU_RANK_SCATTER(512, 16, 1)
 MGPU_RANK_SCATTER(1024, 16, 1)
 MGPU_RANK_SCATTER_EE(512, 16)
 MGPU_RANK_SCATTER_EE(1024, 16)
+MGPU_RANK_SCATTER_KM(512, 16)
+MGPU_RANK_SCATTER_KM(1024, 16)
 MGPU_PACKED(256, 16)
 MGPU_PACKED(512, 16)
 MGPU_PACKED(1024, 16)

@@ -58,10 +58,12 @@ class GpuSort:
         self.data = d
         self.by_ptr = {t.data_ptr(): t for t in self.tensors}
 
-    def run(self, engine, ex=None):
+    def run(self, engine, ex=None, key_type=None):
         torch = torch_mod()
         torch.cuda.synchronize()
-        if ex is None:
+        if key_type is not None:
+            engine.sort_array_typed(self.data, key_type)
+        elif ex is None:
             engine.sort_array(self.data)
         else:
             engine.sort_array_ex(self.data, *ex)

@@ -188,6 +188,100 @@ def test_signed_and_float_keys(engine, oracle, key_type, n):
         assert (np.diff(f.astype(np.float64)) >= 0).all()
 
 
+def _typed_case(kind, n, key_type, seed):
+    """Typed keys (uint32 bit patterns) of one early-exit shape."""
+    rng = np.random.default_rng(seed)
+    bits = _typed_keys(n, key_type, seed)
+    if kind == "sorted_unsigned":  # in order as unsigned words, not as numbers
+        return np.sort(bits)
+    if kind == "sorted_typed":  # already in the sort's order: every pass skipped
+        top = (bits >> np.uint32(31)).astype(bool)
+        flip = np.uint32(0xFFFFFFFF) if key_type == mg.KEYS_FLOAT32 else np.uint32(0x80000000)
+        return bits[np.argsort(bits ^ np.where(top, flip, np.uint32(0x80000000)), kind="stable")]
+    if kind == "low_byte_constant":  # the first pass is skipped
+        return (bits & np.uint32(0xFFFFFF00)) | np.uint32(0x5A)
+    if kind == "top_byte_constant":  # the last pass is skipped
+        return rng.integers(0, 1 << 24, n, dtype=np.int64).astype(np.uint32)
+    return bits
+
+
+@pytest.mark.parametrize("key_type", [mg.KEYS_INT32, mg.KEYS_FLOAT32])
+@pytest.mark.parametrize("kind,n,vc,fb,eb,ee", [
+    ("random", 1, -1, 0, 32, False),
+    ("random", 70_001, -1, 0, 32, False),
+    ("random", 70_001, 1, 0, 32, True),
+    ("random", (1 << 22) + 3, -1, 0, 32, False),
+    ("random", (1 << 22) + 3, 2, 0, 32, True),
+    ("random", (1 << 24) + 5, 1, 0, 32, False),  # lookback chains per XCD
+    ("random", 300_000, 0, 8, 32, False),
+    ("random", 300_000, -1, 0, 8, False),  # one pass maps both ways
+    ("random", 300_000, 1, 4, 21, True),
+    ("sorted_unsigned", 200_000, -1, 0, 32, True),
+    ("sorted_typed", 200_000, -1, 0, 32, True),
+    ("low_byte_constant", 200_000, 1, 0, 32, True),
+    ("top_byte_constant", 200_000, -1, 0, 32, True),
+])
+def test_typed_sort_maps_inside_the_passes(engine, oracle, key_type, kind, n, vc, fb, eb, ee):
+    """sortArrayTyped (the order mapping inside the count and the first and
+    last executed passes) against the oracle's map, stable sort and map back,
+    which is what sortTransformKeys, sortArray, sortTransformKeys compute;
+    with earlyExit, on inputs that skip every pass, the first or the last."""
+    bits = _typed_case(kind, n, key_type, seed=n + 7 * key_type + len(kind))
+    n = bits.size
+    values = [workloads.uniform_keys(n, 500 + j) for j in range(max(vc, 0))]
+    g = GpuSort(bits, values, value_count=vc, first_bit=fb, end_bit=eb)
+    g.data.earlyExit = ee
+    g.run(engine, key_type=key_type)
+    engine.check_device()
+    mapped = oracle.key_transform(bits, key_type, True)
+    rk, rv = oracle.stable_sort(mapped, values, value_count=vc, first_bit=fb, end_bit=eb)
+    assert np.array_equal(g.keys(), oracle.key_transform(rk, key_type, False))
+    for o, r in zip(g.values(), rv):
+        assert np.array_equal(o, r)
+    if kind == "sorted_typed":
+        assert g.data.parity == 0 and np.array_equal(g.keys(), bits)
+
+
+@pytest.mark.parametrize("env", [{"MGPUSORT_RANK": "0"}, {"MGPUSORT_TILE": "256x16"},
+                                 {"MGPUSORT_XCD": "1"}])
+@pytest.mark.parametrize("key_type", [mg.KEYS_INT32, mg.KEYS_FLOAT32])
+def test_typed_sort_other_engines(oracle, env, key_type):
+    """Engines whose tile shape has no key-mapping pass forms (match-mask
+    ranking, 256 x 16 tiles) map the keys in place around the sort instead;
+    chains per XCD at every size use the mapping forms of the _x kernels.
+    Each gives the oracle's result, also with earlyExit skipping passes."""
+    eng = make_engine(env)
+    try:
+        for kind, n, vc, ee in (("random", 100_003, 1, False), ("random", 1 << 20, -1, False),
+                                ("low_byte_constant", 50_000, 2, True),
+                                ("sorted_typed", 30_000, -1, True)):
+            bits = _typed_case(kind, n, key_type, seed=n + key_type)
+            values = [workloads.uniform_keys(n, 900 + j) for j in range(max(vc, 0))]
+            g = GpuSort(bits, values, value_count=vc)
+            g.data.earlyExit = ee
+            g.run(eng, key_type=key_type)
+            eng.check_device()
+            rk, rv = oracle.stable_sort(oracle.key_transform(bits, key_type, True), values,
+                                        value_count=vc)
+            assert np.array_equal(g.keys(), oracle.key_transform(rk, key_type, False)), kind
+            for o, r in zip(g.values(), rv):
+                assert np.array_equal(o, r), kind
+    finally:
+        eng.close()
+
+
+def test_typed_sort_rejects_bad_key_types(engine, oracle):
+    g = GpuSort(workloads.uniform_keys(100, 3))
+    for bad in (-1, 3, 7):
+        with pytest.raises(mg.MgpuError) as e:
+            g.run(engine, key_type=bad)
+        assert e.value.status == mg.SortStatus.INVALID_VALUE
+    # SORT_KEYS_UINT32 is sortArray itself.
+    keys = workloads.uniform_keys(5000, 4)
+    g = GpuSort(keys, value_count=-1).run(engine, key_type=mg.KEYS_UINT32)
+    _check_vs_oracle(oracle, keys, g, value_count=-1)
+
+
 def test_transform_keys_rejects_bad_arguments(engine):
     with pytest.raises(mg.MgpuError):
         engine.transform_keys(0, 10, mg.KEYS_INT32, True)
