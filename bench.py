@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--no-segmented", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-early-exit", action="store_true")
+    ap.add_argument("--no-typed", action="store_true")
     ap.add_argument("--segmented-direct", action="store_true",
                     help="segmented leg through sortSegmentedArray instead of plan replays")
     ap.add_argument("--no-headline", action="store_true",
@@ -199,6 +200,55 @@ def run_workload(torch, eng, name, keys, values, value_count, first_bit, end_bit
     torch.cuda.empty_cache()
     log(f"[bench] {name}: {out['gkeys_per_s']:.2f} Gkeys/s, {out['ms_per_sort']:.3f} ms, "
         f"sort roofline {out['sort_roofline']['frac']:.3f}")
+    return out
+
+
+def typed_bench(torch, eng, log2n, steps):
+    """float32 keys (uniform 32-bit patterns: NaNs, infinities and denormals
+    included) + one value at 2^log2n: sortArrayTyped, which maps the keys
+    inside the count and the first and last passes, against
+    sortTransformKeys + sortArray + sortTransformKeys; wall time per sort over
+    `steps` fresh inputs each, results compared bit for bit."""
+    import mgpu_old_amd as mg
+    from mgpu_old_amd import workloads
+    n = 1 << log2n
+    keys = workloads.uniform_keys_torch(n, 2029, "cuda", torch=torch)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    f32 = mg.KEYS_FLOAT32
+
+    def fused(d):
+        eng.sort_array_typed(d, f32)
+
+    def three_calls(d):
+        eng.transform_keys(d.keys[0], n, f32, True)
+        eng.sort_array(d)
+        eng.transform_keys(d.keys[0], n, f32, False)
+
+    out, results = {"workload": f"float32 2^{log2n} key + value, [0,32)", "n": n}, []
+    for name, fn in (("fused", fused), ("three_calls", three_calls)):
+        log(f"[bench] typed keys: {name}")
+        wl = Workload(torch, keys, [vals], 1, 0, 32, steps + 1)
+        fn(wl.data(0))
+        datas = [wl.data(1 + i) for i in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for d in datas:
+            fn(d)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t0) / steps
+        eng.check_device()
+        k_out, v_out = wl.result(datas[0])
+        results.append((k_out.clone(), v_out.clone()))
+        out[f"{name}_ms"] = sec * 1e3
+        out[f"{name}_gkeys_per_s"] = n / sec / 1e9
+        del wl, datas
+        torch.cuda.empty_cache()
+    out["same_result"] = bool(torch.equal(results[0][0], results[1][0]) and
+                              torch.equal(results[0][1], results[1][1]))
+    log(f"[bench] typed keys: fused {out['fused_ms']:.3f} ms, three calls "
+        f"{out['three_calls_ms']:.3f} ms, same result {out['same_result']}")
+    del keys, vals, results
+    torch.cuda.empty_cache()
     return out
 
 
@@ -579,6 +629,7 @@ def main():
         e.pop("kernels", None)
 
     ee = None if args.no_early_exit else early_exit_bench(torch, eng, args.log2n, 5)
+    typed = None if args.no_typed else typed_bench(torch, eng, args.log2n, 5)
     seg = None if args.no_segmented else segmented_bench(torch, eng, 10, args.segmented_direct)
     served = None if args.no_served else served_bench(torch, eng, args.served_requests)
 
@@ -611,6 +662,7 @@ def main():
         "cpu_baselines": cpu_runs,
         "extra": extra,
         "early_exit": ee,
+        "typed_keys": typed,
         "segmented": seg,
         "served": served,
     }
