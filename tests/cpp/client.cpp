@@ -8,9 +8,9 @@
 //      carry the remaining words plus an index stream as values, with the
 //      caller tracking buffers through `parity`;
 //   3. the additions with no reference counterpart: a segmented sort of 600
-//      arrays (sortSegmentedArray, then a captured plan replayed twice) and
-//      40 requests of mixed sizes, bit widths and value modes through the
-//      sort service.
+//      arrays (sortSegmentedArray, then a captured plan replayed twice), 40
+//      requests of mixed sizes, bit widths and value modes through the sort
+//      service, and int32 and float32 keys through sortArrayTyped.
 // Every result is checked against std::stable_sort on the host.  Prints
 // "CLIENT OK" on success and exits non-zero on the first mismatch.
 #include <hip/hip_runtime.h>
@@ -316,6 +316,55 @@ static int server_test(MgpuSort& engine) {
     return 0;
 }
 
+// sortArrayTyped with index generation: int32 keys, then float32 keys (no
+// NaNs or zeros, so numeric order is the sort's order), each against
+// std::stable_sort by value.
+static int typed_test(MgpuSort& engine) {
+    const int n = 70001;
+    std::mt19937 rng(21);
+    std::normal_distribution<float> normal(0.f, 1000.f);
+    for (int type : {SORT_KEYS_INT32, SORT_KEYS_FLOAT32}) {
+        std::vector<uint32_t> keys(n);
+        for (int i = 0; i < n; ++i) {
+            if (type == SORT_KEYS_INT32) {
+                keys[i] = rng();
+            } else {
+                float f = normal(rng);
+                if (f == 0.f) f = 1.f;
+                std::memcpy(&keys[i], &f, 4);
+            }
+        }
+        uint32_t* dk = dalloc(n);
+        CHECK(dk);
+        CHECK(hipMemcpy(dk, keys.data(), 4 * n, hipMemcpyHostToDevice) == hipSuccess);
+        MgpuSortData data;
+        data.AttachKey(dk);
+        CHECK(data.Alloc(engine, n, SORT_VALUE_INDEX) == SORT_STATUS_SUCCESS);
+        data.firstBit = 0;
+        data.endBit = 32;
+        CHECK(sortArrayTyped(engine, &data, type) == SORT_STATUS_SUCCESS);
+        std::vector<uint32_t> ok(n), oi(n);
+        CHECK(hipMemcpy(ok.data(), data.pKeys[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+        CHECK(hipMemcpy(oi.data(), data.pValues1[0], 4 * n, hipMemcpyDeviceToHost) == hipSuccess);
+        std::vector<uint32_t> perm(n);
+        std::iota(perm.begin(), perm.end(), 0u);
+        std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+            if (type == SORT_KEYS_INT32) return (int32_t)keys[a] < (int32_t)keys[b];
+            float fa, fb;
+            std::memcpy(&fa, &keys[a], 4);
+            std::memcpy(&fb, &keys[b], 4);
+            return fa < fb;
+        });
+        for (int i = 0; i < n; ++i) {
+            CHECK(ok[i] == keys[perm[i]]);
+            CHECK(oi[i] == perm[i]);
+        }
+        data.Reset();
+        (void)hipFree(dk);
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     const char* path = argc > 1 ? argv[1] : nullptr;
     try {
@@ -324,6 +373,7 @@ int main(int argc, char** argv) {
         if (bwt_style_test(engine)) return 1;
         if (segmented_test(engine)) return 1;
         if (server_test(engine)) return 1;
+        if (typed_test(engine)) return 1;
     } catch (const MgpuException& e) {
         std::printf("MgpuException: %s\n", e.what());
         return 2;
