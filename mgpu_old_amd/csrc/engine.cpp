@@ -200,6 +200,8 @@ sortStatus_t mgpu_lane_init(SortLane* l) {
     l->digitStart = l->hist + kLaneHistWords;
     l->tileCounters = l->digitStart + kLaneHistWords;
     l->segCount = l->tileCounters + MGPU_LANE_COUNTERS;
+    static_assert(MGPU_LANE_COUNTERS % 2 == 0 && kLaneHistWords % 2 == 0,
+                  "segCount takes 64-bit atomics: it starts on an 8-byte boundary");
     l->histDirty = false;
     return SORT_STATUS_SUCCESS;
 }

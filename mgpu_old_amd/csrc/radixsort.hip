@@ -1291,10 +1291,15 @@ __device__ void rank_scatter_body(const Args& a) {
     // (each needs a workgroup on its XCD).
     if constexpr (XSINGLE) {
         if (count_next) {
+            // Two digits' counts per 64-bit atomic (half the device-scope
+            // atomics, which every workgroup issues at the same moment): a
+            // word's total is at most n < 2^31, so the low half never
+            // carries into the high one.  seg_count_out is 8-byte aligned.
             __syncthreads();
-            for (uint32_t i = tid; i < NSEG * RADIX; i += BLOCK) {
-                const uint32_t c = s_next[i];
-                if (c) __hip_atomic_fetch_add(&a.seg_count_out[i], c, __ATOMIC_RELAXED,
+            uint64_t* const out2 = reinterpret_cast<uint64_t*>(a.seg_count_out);
+            for (uint32_t i = tid; i < NSEG * RADIX / 2; i += BLOCK) {
+                const uint64_t c = (uint64_t)s_next[2 * i + 1] << 32 | s_next[2 * i];
+                if (c) __hip_atomic_fetch_add(&out2[i], c, __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
             }
         }
