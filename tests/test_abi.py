@@ -146,6 +146,8 @@ def test_null_arguments_rejected_without_device():
     d = mg.SortData()
     assert L.sortArray(None, ctypes.byref(d)) == mg.SortStatus.INVALID_VALUE
     assert L.sortArrayEx(None, ctypes.byref(d), 256, 8, 6, False) == mg.SortStatus.INVALID_VALUE
+    for key_type in (mg.KEYS_UINT32, mg.KEYS_INT32, mg.KEYS_FLOAT32, 3, -1):
+        assert L.sortArrayTyped(None, ctypes.byref(d), key_type) == mg.SortStatus.INVALID_VALUE
     assert L.sortIncEngine(None) == mg.SortStatus.INVALID_VALUE
     assert L.sortReleaseEngine(None) == mg.SortStatus.SUCCESS
     assert L.sortDestroyEngine(None) == mg.SortStatus.SUCCESS
