@@ -242,6 +242,49 @@ def test_typed_sort_maps_inside_the_passes(engine, oracle, key_type, kind, n, vc
         assert g.data.parity == 0 and np.array_equal(g.keys(), bits)
 
 
+@pytest.mark.parametrize("key_type,vc", [(mg.KEYS_FLOAT32, 1), (mg.KEYS_INT32, -1)])
+def test_typed_sort_equals_three_calls_at_2p26(engine, key_type, vc):
+    """At 2^26 keys (lookback chains per XCD), sortArrayTyped gives the same
+    bits as sortTransformKeys, sortArray, sortTransformKeys on the same input
+    (both compared on the device); the smaller typed tests tie both to the
+    oracle."""
+    import torch
+    n = 1 << 26
+    keys = workloads.uniform_keys_torch(n, 260 + key_type, "cuda", torch=torch)
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    outs = []
+    for fused in (True, False):
+        k0, k1 = keys.clone(), torch.empty_like(keys)
+        v0, v1 = (vals.clone() if vc == 1 else torch.empty_like(vals)), torch.empty_like(vals)
+        d = mg.SortData()
+        d.maxElements = d.numElements = n
+        d.valueCount = vc
+        d.firstBit, d.endBit = 0, 32
+        d.keys[0], d.keys[1] = k0.data_ptr(), k1.data_ptr()
+        d.values[0][0], d.values[0][1] = v0.data_ptr(), v1.data_ptr()
+        torch.cuda.synchronize()
+        if fused:
+            engine.sort_array_typed(d, key_type)
+        else:
+            engine.transform_keys(d.keys[0], n, key_type, True)
+            engine.sort_array(d)
+            engine.transform_keys(d.keys[0], n, key_type, False)
+        torch.cuda.synchronize()
+        engine.check_device()
+        by = {t.data_ptr(): t for t in (k0, k1, v0, v1)}
+        outs.append((by[d.keys[0]], by[d.values[0][0]], d.parity))
+    assert outs[0][2] == outs[1][2]
+    assert torch.equal(outs[0][0], outs[1][0]) and torch.equal(outs[0][1], outs[1][1])
+    # and the result is in the type's order
+    k = outs[0][0]
+    if key_type == mg.KEYS_INT32:
+        assert bool((k[1:] >= k[:-1]).all())
+    else:
+        f = k.view(torch.float32)
+        f = f[~torch.isnan(f)]
+        assert bool((f[1:] >= f[:-1]).all())
+
+
 @pytest.mark.parametrize("env", [{"MGPUSORT_RANK": "0"}, {"MGPUSORT_TILE": "256x16"},
                                  {"MGPUSORT_XCD": "1"}])
 @pytest.mark.parametrize("key_type", [mg.KEYS_INT32, mg.KEYS_FLOAT32])
