@@ -202,8 +202,11 @@ constexpr uint32_t kCUsPerLargeTile = 2;
 // (2^24 keys in 16384-key tiles on 8 XCDs).
 constexpr uint32_t kXsegMinTilesPerSeg = 128;
 // digit_count with lane-column histograms (one 1024-thread workgroup per CU)
-// from this many keys up.
-constexpr uint32_t kCountColsMinKeys = 1u << 22;
+// from this many keys up; the per-wave form (256 threads, up to 8 per CU)
+// below.  Measured per launch: 2^20 keys 11.7 against 12.8 us, 2^21 12.2
+// against 15.2, 2^22 13.8 against 20.0; equal at 2^16-2^19; the per-wave
+// form faster at 2^12 and below (7.6 against 8.2 us).
+constexpr uint32_t kCountColsMinKeys = 1u << 20;
 const PassVariant& mgpu_pick_variant(const sortEngine_d* e, uint64_t keys);
 
 // Value classes of the sortArray tuning table.
